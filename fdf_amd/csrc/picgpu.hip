@@ -1,0 +1,883 @@
+// picgpu.hip -- C-ABI of the gfx950 particle loop (include/picgpu.h).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "kernels.hpp"
+
+using namespace picgpu;
+
+namespace picgpu {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+hipEvent_t take_event(picgpu_ctx *c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void timer_begin(picgpu_ctx *c, const char *name, TimerRec &rec) {
+  rec.a = rec.b = nullptr;
+  if (!c->timing) return;
+  rec.name = name;
+  rec.a = take_event(c);
+  rec.b = take_event(c);
+  if (rec.a) (void)hipEventRecord(rec.a, c->stream);
+}
+
+void timer_end(picgpu_ctx *c, TimerRec &rec) {
+  if (!c->timing || !rec.a || !rec.b) return;
+  (void)hipEventRecord(rec.b, c->stream);
+  c->pending.push_back(rec);
+}
+
+int flush_timers(picgpu_ctx *c) {
+  if (c->pending.empty()) return PICGPU_OK;
+  PG_HIP(hipStreamSynchronize(c->stream));
+  for (auto &r : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      auto &acc = c->timing_acc[r.name];
+      acc.first += ms;
+      acc.second += 1;
+    }
+    c->event_pool.push_back(r.a);
+    c->event_pool.push_back(r.b);
+  }
+  c->pending.clear();
+  return PICGPU_OK;
+}
+
+}  // namespace picgpu
+
+static inline int64_t nblocks(int64_t n, int b = kBlock) { return (n + b - 1) / b; }
+
+template <typename F>
+static int dispatch_real(picgpu_ctx *c, F &&f) {
+  if (c->opt.precision == PICGPU_FP32) return f((float *)nullptr);
+  return f((double *)nullptr);
+}
+
+static int counters_reset(picgpu_ctx *c) {
+  PG_CHECK(c->counters.ensure(sizeof(unsigned long long) * C_NUM));
+  PG_HIP(hipMemsetAsync(c->counters.ptr, 0, sizeof(unsigned long long) * C_NUM, c->stream));
+  return PICGPU_OK;
+}
+
+static int counters_read(picgpu_ctx *c, unsigned long long *h) {
+  PG_HIP(hipMemcpyAsync(h, c->counters.ptr, sizeof(unsigned long long) * C_NUM, hipMemcpyDeviceToHost, c->stream));
+  PG_HIP(hipStreamSynchronize(c->stream));
+  return PICGPU_OK;
+}
+
+static int ensure_swarm(picgpu_ctx *c, SwarmBufs &s, int64_t n) {
+  size_t m = (size_t)(n > 0 ? n : 1);
+  PG_CHECK(s.pos.ensure(m * 3 * sizeof(double)));
+  PG_CHECK(s.vel.ensure(m * 3 * c->rsize));
+  PG_CHECK(s.w.ensure(m * 3 * c->rsize));
+  PG_CHECK(s.cell.ensure(m * sizeof(int32_t)));
+  PG_CHECK(s.status.ensure(m * sizeof(int32_t)));
+  PG_CHECK(s.pid.ensure(m * sizeof(int64_t)));
+  PG_CHECK(s.P.ensure(m * c->rsize));
+  return PICGPU_OK;
+}
+
+static void swap_swarm(picgpu_ctx *c) { std::swap(c->sw, c->alt); }
+
+static int need_grid(picgpu_ctx *c) {
+  if (!c->have_grid) {
+    set_error("grid not uploaded");
+    return PICGPU_E_STATE;
+  }
+  return PICGPU_OK;
+}
+
+static GridView grid_view(picgpu_ctx *c) { return GridView{c->coor.as<double>(), c->IM, c->JM, c->KM}; }
+
+// ---------------------------------------------------------------------------
+// lifetime
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void picgpu_default_options(picgpu_options *o) {
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->precision = PICGPU_FP64;
+  o->use_mlp_guess = 0;
+  o->max_traversal = 1000;
+  o->distance_threshold = 1e-11;
+  o->weight_clamp = 1.0e-14;
+  o->bbox_tolerance = 1e-6;
+  o->device = 0;
+  o->collect_walk_stats = 0;
+}
+
+const char *picgpu_last_error(void) { return g_err; }
+
+int picgpu_create(picgpu_ctx **out, const picgpu_options *opt) {
+  if (!out) {
+    set_error("null ctx pointer");
+    return PICGPU_E_ARG;
+  }
+  *out = nullptr;
+  picgpu_ctx *c = new picgpu_ctx();
+  if (opt)
+    c->opt = *opt;
+  else
+    picgpu_default_options(&c->opt);
+  if (c->opt.precision != PICGPU_FP64 && c->opt.precision != PICGPU_FP32) {
+    delete c;
+    set_error("unknown precision mode %d", opt ? opt->precision : -1);
+    return PICGPU_E_ARG;
+  }
+  if (c->opt.max_traversal <= 0) {
+    delete c;
+    set_error("max_traversal must be positive");
+    return PICGPU_E_ARG;
+  }
+  c->rsize = c->opt.precision == PICGPU_FP32 ? sizeof(float) : sizeof(double);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    delete c;
+    set_error("no HIP device available");
+    return PICGPU_E_HIP;
+  }
+  if (c->opt.device < 0 || c->opt.device >= ndev) {
+    delete c;
+    set_error("device %d out of range (%d devices)", c->opt.device, ndev);
+    return PICGPU_E_ARG;
+  }
+  c->device = c->opt.device;
+  if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    set_error("HIP stream creation failed");
+    return PICGPU_E_HIP;
+  }
+  *out = c;
+  return PICGPU_OK;
+}
+
+int picgpu_destroy(picgpu_ctx *c) {
+  if (!c) return PICGPU_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto &r : c->pending) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  DevBuf *bufs[] = {&c->coor, &c->ucat, &c->corner, &c->rank, &c->keep, &c->walk_steps, &c->guess, &c->pos0,
+                    &c->vel0, &c->cell_count, &c->cell_start, &c->pavg, &c->face[0], &c->face[1], &c->face[2],
+                    &c->scan_tmp, &c->counters, &c->mlp_master, &c->mlp_bf16, &c->mlp_adam_m, &c->mlp_adam_v,
+                    &c->mlp_grad, &c->mlp_partial, &c->mlp_x, &c->mlp_t, &c->mlp_y};
+  for (DevBuf *b : bufs) b->release();
+  SwarmBufs *sws[] = {&c->sw, &c->alt};
+  for (SwarmBufs *s : sws) {
+    s->pos.release();
+    s->vel.release();
+    s->w.release();
+    s->cell.release();
+    s->status.release();
+    s->pid.release();
+    s->P.release();
+  }
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PICGPU_OK;
+}
+
+int picgpu_synchronize(picgpu_ctx *c) {
+  if (!c) return PICGPU_E_ARG;
+  PG_HIP(hipStreamSynchronize(c->stream));
+  return PICGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// grid and Eulerian input
+// ---------------------------------------------------------------------------
+int picgpu_upload_grid(picgpu_ctx *c, int64_t IM, int64_t JM, int64_t KM, const double *coor) {
+  if (!c || !coor || IM <= 0 || JM <= 0 || KM <= 0) {
+    set_error("upload_grid: bad arguments");
+    return PICGPU_E_ARG;
+  }
+  if (IM * JM * KM >= (int64_t)INT32_MAX - 2 || (IM + 1) * (JM + 1) * (KM + 1) >= ((int64_t)1 << 40)) {
+    set_error("upload_grid: grid too large for 32-bit packed cell indices");
+    return PICGPU_E_UNSUPPORTED;
+  }
+  PG_HIP(hipSetDevice(c->device));
+  c->IM = (int)IM;
+  c->JM = (int)JM;
+  c->KM = (int)KM;
+  c->ncell = IM * JM * KM;
+  c->nnode = (IM + 1) * (JM + 1) * (KM + 1);
+  PG_CHECK(c->coor.ensure(sizeof(double) * 3 * (size_t)c->nnode));
+  PG_HIP(hipMemcpy(c->coor.ptr, coor, sizeof(double) * 3 * (size_t)c->nnode, hipMemcpyHostToDevice));
+  // padded bounding box (grid.c:762-883 on one rank)
+  for (int a = 0; a < 3; ++a) {
+    c->bmin[a] = 1.7976931348623157e308;
+    c->bmax[a] = -1.7976931348623157e308;
+  }
+  for (int64_t q = 0; q < c->nnode; ++q)
+    for (int a = 0; a < 3; ++a) {
+      double x = coor[3 * q + a];
+      if (x < c->bmin[a]) c->bmin[a] = x;
+      if (x > c->bmax[a]) c->bmax[a] = x;
+    }
+  for (int a = 0; a < 3; ++a) {
+    c->bmin[a] = c->bmin[a] - c->opt.bbox_tolerance;
+    c->bmax[a] = c->bmax[a] + c->opt.bbox_tolerance;
+  }
+  c->have_grid = true;
+  c->have_ucat = false;
+  c->sorted = false;
+  c->have_pavg = c->have_faces = false;
+  return PICGPU_OK;
+}
+
+int picgpu_upload_ucat(picgpu_ctx *c, const double *ucat) {
+  if (!c || !ucat) return PICGPU_E_ARG;
+  PG_CHECK(need_grid(c));
+  PG_HIP(hipSetDevice(c->device));
+  const int64_t nx = c->IM + 1, ny = c->JM + 1;
+  PG_CHECK(c->ucat.ensure(c->rsize * 3 * (size_t)c->ncell));
+  PG_CHECK(c->corner.ensure(c->rsize * 3 * (size_t)c->nnode));
+  return dispatch_real(c, [&](auto *tag) -> int {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    std::vector<R> h((size_t)c->ncell * 3);
+    for (int64_t k = 0; k < c->KM; ++k)
+      for (int64_t j = 0; j < c->JM; ++j)
+        for (int64_t i = 0; i < c->IM; ++i) {
+          const double *u = ucat + 3 * ((k * ny + j) * nx + i);
+          R *o = h.data() + 3 * ((k * c->JM + j) * c->IM + i);
+          o[0] = (R)u[0];
+          o[1] = (R)u[1];
+          o[2] = (R)u[2];
+        }
+    PG_HIP(hipMemcpy(c->ucat.ptr, h.data(), sizeof(R) * h.size(), hipMemcpyHostToDevice));
+    c->have_ucat = true;
+    return PICGPU_OK;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// swarm I/O
+// ---------------------------------------------------------------------------
+int picgpu_upload_particles(picgpu_ctx *c, int64_t n, const double *pos, const double *vel, const int64_t *cell,
+                            const double *weight, const int64_t *status, const int64_t *pid, const double *P) {
+  if (!c || n < 0 || (n > 0 && (!pos || !cell || !status || !pid))) {
+    set_error("upload_particles: bad arguments");
+    return PICGPU_E_ARG;
+  }
+  PG_CHECK(need_grid(c));
+  if (n >= (int64_t)INT32_MAX) {
+    set_error("upload_particles: more than 2^31-1 particles");
+    return PICGPU_E_UNSUPPORTED;
+  }
+  PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(ensure_swarm(c, c->sw, n));
+  c->n = n;
+  c->sorted = false;
+  if (n == 0) return PICGPU_OK;
+  PG_HIP(hipMemcpy(c->sw.pos.ptr, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice));
+  PG_HIP(hipMemcpy(c->sw.pid.ptr, pid, sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice));
+  {
+    std::vector<int32_t> hc((size_t)n), hs((size_t)n);
+    for (int64_t p = 0; p < n; ++p) {
+      int64_t i = cell[3 * p], j = cell[3 * p + 1], k = cell[3 * p + 2];
+      bool ok = i >= 0 && j >= 0 && k >= 0 && i < c->IM && j < c->JM && k < c->KM;
+      hc[p] = ok ? (int32_t)(i + (int64_t)c->IM * (j + (int64_t)c->JM * k)) : -1;
+      hs[p] = (int32_t)status[p];
+    }
+    PG_HIP(hipMemcpy(c->sw.cell.ptr, hc.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice));
+    PG_HIP(hipMemcpy(c->sw.status.ptr, hs.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice));
+  }
+  return dispatch_real(c, [&](auto *tag) -> int {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    std::vector<R> h((size_t)n * 3);
+    for (int64_t q = 0; q < 3 * n; ++q) h[q] = vel ? (R)vel[q] : (R)0;
+    PG_HIP(hipMemcpy(c->sw.vel.ptr, h.data(), sizeof(R) * 3 * (size_t)n, hipMemcpyHostToDevice));
+    for (int64_t q = 0; q < 3 * n; ++q) h[q] = weight ? (R)weight[q] : (R)0;
+    PG_HIP(hipMemcpy(c->sw.w.ptr, h.data(), sizeof(R) * 3 * (size_t)n, hipMemcpyHostToDevice));
+    for (int64_t q = 0; q < n; ++q) h[q] = P ? (R)P[q] : (R)101325.0;
+    PG_HIP(hipMemcpy(c->sw.P.ptr, h.data(), sizeof(R) * (size_t)n, hipMemcpyHostToDevice));
+    return PICGPU_OK;
+  });
+}
+
+int picgpu_num_particles(picgpu_ctx *c, int64_t *n) {
+  if (!c || !n) return PICGPU_E_ARG;
+  *n = c->n;
+  return PICGPU_OK;
+}
+
+int picgpu_download_particles(picgpu_ctx *c, double *pos, double *vel, int64_t *cell, double *weight,
+                              int64_t *status, int64_t *pid) {
+  if (!c) return PICGPU_E_ARG;
+  const int64_t n = c->n;
+  PG_HIP(hipSetDevice(c->device));
+  PG_HIP(hipStreamSynchronize(c->stream));
+  if (n == 0) return PICGPU_OK;
+  if (pos) PG_HIP(hipMemcpy(pos, c->sw.pos.ptr, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost));
+  if (pid) PG_HIP(hipMemcpy(pid, c->sw.pid.ptr, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToHost));
+  if (cell || status) {
+    std::vector<int32_t> hc((size_t)n), hs((size_t)n);
+    PG_HIP(hipMemcpy(hc.data(), c->sw.cell.ptr, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost));
+    PG_HIP(hipMemcpy(hs.data(), c->sw.status.ptr, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost));
+    for (int64_t p = 0; p < n; ++p) {
+      if (cell) {
+        int32_t v = hc[p];
+        if (v < 0) {
+          cell[3 * p] = cell[3 * p + 1] = cell[3 * p + 2] = -1;
+        } else {
+          cell[3 * p] = v % c->IM;
+          cell[3 * p + 1] = (v / c->IM) % c->JM;
+          cell[3 * p + 2] = v / ((int64_t)c->IM * c->JM);
+        }
+      }
+      if (status) status[p] = hs[p];
+    }
+  }
+  return dispatch_real(c, [&](auto *tag) -> int {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    std::vector<R> h((size_t)n * 3);
+    if (vel) {
+      PG_HIP(hipMemcpy(h.data(), c->sw.vel.ptr, sizeof(R) * 3 * (size_t)n, hipMemcpyDeviceToHost));
+      for (int64_t q = 0; q < 3 * n; ++q) vel[q] = (double)h[q];
+    }
+    if (weight) {
+      PG_HIP(hipMemcpy(h.data(), c->sw.w.ptr, sizeof(R) * 3 * (size_t)n, hipMemcpyDeviceToHost));
+      for (int64_t q = 0; q < 3 * n; ++q) weight[q] = (double)h[q];
+    }
+    return PICGPU_OK;
+  });
+}
+
+int picgpu_download_walk_steps(picgpu_ctx *c, int32_t *steps) {
+  if (!c || !steps) return PICGPU_E_ARG;
+  if (!c->opt.collect_walk_stats || !c->walk_steps.ptr || c->walk_steps.bytes < sizeof(int32_t) * (size_t)c->n) {
+    set_error("walk statistics not collected (collect_walk_stats=0 or no locate yet)");
+    return PICGPU_E_STATE;
+  }
+  PG_HIP(hipStreamSynchronize(c->stream));
+  if (c->n) PG_HIP(hipMemcpy(steps, c->walk_steps.ptr, sizeof(int32_t) * (size_t)c->n, hipMemcpyDeviceToHost));
+  return PICGPU_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// stage implementations
+// ---------------------------------------------------------------------------
+static int do_reset_advect(picgpu_ctx *c, double dt, int reset, int advect) {
+  if (c->n == 0) return PICGPU_OK;
+  return dispatch_real(c, [&](auto *tag) -> int {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    PG_LAUNCH(c, "reset_advect", nblocks(c->n), kBlock, k_reset_advect<R>, c->n, c->sw.pos.as<double>(),
+              c->sw.vel.as<R>(), c->sw.status.as<int32_t>(), dt, reset, advect);
+    c->sorted = false;
+    return PICGPU_OK;
+  });
+}
+
+static int do_locate(picgpu_ctx *c, int skip_lost) {
+  PG_CHECK(need_grid(c));
+  PG_CHECK(counters_reset(c));
+  const int64_t n = c->n;
+  if (n > 0) {
+    int32_t *guess = nullptr;
+    if (c->opt.use_mlp_guess) {
+      if (!c->have_mlp) {
+        set_error("use_mlp_guess=1 but no MLP parameters are set");
+        return PICGPU_E_STATE;
+      }
+      PG_CHECK(c->guess.ensure(sizeof(int32_t) * (size_t)n));
+      PG_CHECK(mlp_guess_device(c, n, c->sw.pos.as<double>(), c->sw.cell.as<int32_t>(), c->guess.as<int32_t>()));
+      guess = c->guess.as<int32_t>();
+    }
+    int32_t *steps = nullptr;
+    if (c->opt.collect_walk_stats) {
+      PG_CHECK(c->walk_steps.ensure(sizeof(int32_t) * (size_t)n));
+      steps = c->walk_steps.as<int32_t>();
+    }
+    const int64_t chunk = 256;  // particles per wavefront
+    const int64_t waves = (n + chunk - 1) / chunk;
+    const int64_t blocks = (waves * 64 + kBlock - 1) / kBlock;
+    int r = dispatch_real(c, [&](auto *tag) -> int {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      LocateArgs<R> a;
+      a.g = grid_view(c);
+      a.n = n;
+      a.chunk = chunk;
+      a.pos = c->sw.pos.as<double>();
+      a.cell = c->sw.cell.as<int32_t>();
+      a.w = c->sw.w.as<R>();
+      a.status = c->sw.status.as<int32_t>();
+      a.guess = guess;
+      a.steps_out = steps;
+      a.counters = c->counters.as<unsigned long long>();
+      a.max_traversal = c->opt.max_traversal;
+      a.thresh_scale = c->opt.distance_threshold;
+      a.clamp = (R)c->opt.weight_clamp;
+      a.skip_lost = skip_lost;
+      if (sizeof(R) == sizeof(double)) {
+        PG_LAUNCH(c, "locate_fp64", blocks, kBlock, (k_locate<R, true>), a);
+      } else {
+        PG_LAUNCH(c, "locate_fp32", blocks, kBlock, (k_locate<R, false>), a);
+      }
+      return PICGPU_OK;
+    });
+    PG_CHECK(r);
+  }
+  unsigned long long h[C_NUM];
+  PG_CHECK(counters_read(c, h));
+  if (h[C_ERROR]) {
+    set_error("degenerate cell face (|n| < 1e-12) met during the walk");
+    return PICGPU_E_DEGENERATE;
+  }
+  c->stats.walk_iterations = (int64_t)h[C_WALK_ITERS];
+  c->stats.walk_max = (int64_t)h[C_WALK_MAX];
+  c->stats.fp64_fallbacks = (int64_t)h[C_FALLBACKS];
+  c->stats.located = (int64_t)h[C_LOCATED];
+  c->stats.lost = (int64_t)h[C_LOST];
+  c->sorted = false;
+  return PICGPU_OK;
+}
+
+// Stable compaction of the swarm keeping particles by the flags.
+static int do_remove(picgpu_ctx *c, int check_lost, int check_box, int64_t *removed_lost, int64_t *removed_oob) {
+  const int64_t n = c->n;
+  if (removed_lost) *removed_lost = 0;
+  if (removed_oob) *removed_oob = 0;
+  if (n == 0) return PICGPU_OK;
+  PG_CHECK(counters_reset(c));
+  PG_CHECK(c->keep.ensure(sizeof(int32_t) * (size_t)(2 * n + 2)));
+  int32_t *keep = c->keep.as<int32_t>();
+  int32_t *off = keep + n + 1;
+  PG_LAUNCH(c, "keep_flags", nblocks(n), kBlock, k_keep_flags, n, c->sw.pos.as<double>(), c->sw.status.as<int32_t>(),
+            check_lost, check_box, c->bmin[0], c->bmin[1], c->bmin[2], c->bmax[0], c->bmax[1], c->bmax[2], keep,
+            c->counters.as<unsigned long long>());
+  PG_CHECK(scan_exclusive_i32(c, keep, off, n));
+  int32_t total = 0;
+  PG_HIP(hipMemcpyAsync(&total, off + n, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  unsigned long long h[C_NUM];
+  PG_CHECK(counters_read(c, h));  // synchronises
+  if (removed_lost) *removed_lost = (int64_t)h[C_REMOVED_LOST];
+  if (removed_oob) *removed_oob = (int64_t)h[C_REMOVED_OOB];
+  if (total == n) return PICGPU_OK;
+  PG_CHECK(ensure_swarm(c, c->alt, n));
+  int r = dispatch_real(c, [&](auto *tag) -> int {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    PG_LAUNCH(c, "compact", nblocks(n), kBlock, k_compact<R>, n, keep, off, (int64_t)0, c->sw.pos.as<double>(), c->sw.vel.as<R>(),
+              c->sw.w.as<R>(), c->sw.cell.as<int32_t>(), c->sw.status.as<int32_t>(), c->sw.pid.as<int64_t>(),
+              c->sw.P.as<R>(), c->alt.pos.as<double>(), c->alt.vel.as<R>(), c->alt.w.as<R>(),
+              c->alt.cell.as<int32_t>(), c->alt.status.as<int32_t>(), c->alt.pid.as<int64_t>(), c->alt.P.as<R>());
+    return PICGPU_OK;
+  });
+  PG_CHECK(r);
+  swap_swarm(c);
+  c->n = total;
+  c->sorted = false;
+  return PICGPU_OK;
+}
+
+static int do_center_to_corner(picgpu_ctx *c) {
+  if (!c->have_ucat) {
+    set_error("ucat not uploaded");
+    return PICGPU_E_STATE;
+  }
+  return dispatch_real(c, [&](auto *tag) -> int {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    PG_LAUNCH(c, "center_to_corner", nblocks(c->nnode), kBlock, k_center_to_corner<R>, c->IM, c->JM, c->KM,
+              c->ucat.as<R>(), c->corner.as<R>());
+    return PICGPU_OK;
+  });
+}
+
+static int do_interp(picgpu_ctx *c) {
+  if (c->n == 0) return PICGPU_OK;
+  return dispatch_real(c, [&](auto *tag) -> int {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    PG_LAUNCH(c, "interp", nblocks(c->n), kBlock, k_interp<R>, c->n, c->IM, c->JM, c->KM, c->corner.as<R>(),
+              c->sw.cell.as<int32_t>(), c->sw.w.as<R>(), c->sw.vel.as<R>());
+    return PICGPU_OK;
+  });
+}
+
+// Counting sort by cell; particles without a cell go to a last bucket.
+static int do_sort(picgpu_ctx *c) {
+  const int64_t n = c->n;
+  const int64_t nb = c->ncell + 1;  // + unlocated bucket
+  PG_CHECK(c->cell_count.ensure(sizeof(int32_t) * (size_t)(nb + 1)));
+  PG_CHECK(c->cell_start.ensure(sizeof(int32_t) * (size_t)(nb + 1)));
+  PG_HIP(hipMemsetAsync(c->cell_count.ptr, 0, sizeof(int32_t) * (size_t)(nb + 1), c->stream));
+  if (n > 0) {
+    PG_CHECK(c->rank.ensure(sizeof(int32_t) * (size_t)n));
+    PG_CHECK(ensure_swarm(c, c->alt, n));
+    // unlocated particles are keyed to bucket ncell by a temporary cell view
+    PG_LAUNCH(c, "cell_histogram", nblocks(n), kBlock, k_cell_histogram, n, c->sw.cell.as<int32_t>(),
+              c->cell_count.as<int32_t>(), c->rank.as<int32_t>());
+  }
+  PG_CHECK(scan_exclusive_i32(c, c->cell_count.as<int32_t>(), c->cell_start.as<int32_t>(), c->ncell));
+  if (n > 0) {
+    int r = dispatch_real(c, [&](auto *tag) -> int {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      PG_LAUNCH(c, "sort_scatter", nblocks(n), kBlock, k_sort_scatter<R>, n, c->rank.as<int32_t>(),
+                c->cell_start.as<int32_t>(), c->sw.pos.as<double>(), c->sw.vel.as<R>(), c->sw.w.as<R>(),
+                c->sw.cell.as<int32_t>(), c->sw.status.as<int32_t>(), c->sw.pid.as<int64_t>(), c->sw.P.as<R>(),
+                c->alt.pos.as<double>(), c->alt.vel.as<R>(), c->alt.w.as<R>(), c->alt.cell.as<int32_t>(),
+                c->alt.status.as<int32_t>(), c->alt.pid.as<int64_t>(), c->alt.P.as<R>());
+      return PICGPU_OK;
+    });
+    PG_CHECK(r);
+    // particles without a cell are appended after the located ones, in order
+    int32_t located = 0;
+    PG_HIP(hipMemcpyAsync(&located, c->cell_start.as<int32_t>() + c->ncell, sizeof(int32_t), hipMemcpyDeviceToHost,
+                          c->stream));
+    PG_HIP(hipStreamSynchronize(c->stream));
+    if (located < n) {
+      // stable compaction of the unlocated tail (rare: LOST particles kept by the caller)
+      PG_CHECK(c->keep.ensure(sizeof(int32_t) * (size_t)(2 * n + 2)));
+      int32_t *keep = c->keep.as<int32_t>();
+      int32_t *off = keep + n + 1;
+      PG_LAUNCH(c, "keep_unlocated", nblocks(n), kBlock, k_keep_unlocated, n, c->sw.cell.as<int32_t>(), keep);
+      PG_CHECK(scan_exclusive_i32(c, keep, off, n));
+      int r2 = dispatch_real(c, [&](auto *tag) -> int {
+        using R = std::remove_pointer_t<decltype(tag)>;
+        PG_LAUNCH(c, "compact_tail", nblocks(n), kBlock, k_compact<R>, n, keep, off, (int64_t)located,
+                  c->sw.pos.as<double>(), c->sw.vel.as<R>(), c->sw.w.as<R>(), c->sw.cell.as<int32_t>(),
+                  c->sw.status.as<int32_t>(), c->sw.pid.as<int64_t>(), c->sw.P.as<R>(), c->alt.pos.as<double>(),
+                  c->alt.vel.as<R>(), c->alt.w.as<R>(), c->alt.cell.as<int32_t>(), c->alt.status.as<int32_t>(),
+                  c->alt.pid.as<int64_t>(), c->alt.P.as<R>());
+        return PICGPU_OK;
+      });
+      PG_CHECK(r2);
+    }
+    swap_swarm(c);
+  }
+  c->sorted = true;
+  return PICGPU_OK;
+}
+
+static int do_scatter_P(picgpu_ctx *c) {
+  PG_CHECK(c->pavg.ensure(c->rsize * (size_t)c->ncell));
+  int r = dispatch_real(c, [&](auto *tag) -> int {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    PG_LAUNCH(c, "scatter_P", nblocks(c->ncell), kBlock, k_scatter_P<R>, c->ncell, c->cell_start.as<int32_t>(),
+              c->sw.pid.as<int64_t>(), c->sw.P.as<R>(), c->pavg.as<R>());
+    return PICGPU_OK;
+  });
+  PG_CHECK(r);
+  c->have_pavg = true;
+  return PICGPU_OK;
+}
+
+static int64_t nface(picgpu_ctx *c, int dir) {
+  return (int64_t)(c->IM + (dir == 0)) * (c->JM + (dir == 1)) * (c->KM + (dir == 2));
+}
+
+static int do_deposit_faces(picgpu_ctx *c) {
+  for (int dir = 0; dir < 3; ++dir) PG_CHECK(c->face[dir].ensure(c->rsize * 4 * (size_t)nface(c, dir)));
+  int r = dispatch_real(c, [&](auto *tag) -> int {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    for (int dir = 0; dir < 3; ++dir) {
+      PG_LAUNCH(c, "deposit_faces", nblocks(nface(c, dir)), kBlock, k_deposit_faces<R>, dir, c->IM, c->JM, c->KM,
+                c->cell_start.as<int32_t>(), c->sw.pid.as<int64_t>(), c->sw.w.as<R>(), c->sw.vel.as<R>(),
+                c->face[dir].as<R>());
+    }
+    return PICGPU_OK;
+  });
+  PG_CHECK(r);
+  c->have_faces = true;
+  return PICGPU_OK;
+}
+
+static int do_rk2(picgpu_ctx *c, double dt) {
+  const int64_t n = c->n;
+  PG_CHECK(c->pos0.ensure(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1)));
+  PG_CHECK(c->vel0.ensure(c->rsize * 3 * (size_t)(n > 0 ? n : 1)));
+  PG_CHECK(do_reset_advect(c, 0.0, 1, 0));
+  if (n > 0) {
+    int r = dispatch_real(c, [&](auto *tag) -> int {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      PG_LAUNCH(c, "rk2_predict", nblocks(n), kBlock, k_rk2_predict<R>, n, c->sw.pos.as<double>(), c->sw.vel.as<R>(),
+                c->sw.status.as<int32_t>(), c->pos0.as<double>(), c->vel0.as<R>(), dt);
+      return PICGPU_OK;
+    });
+    PG_CHECK(r);
+  }
+  PG_CHECK(do_locate(c, 1));
+  PG_CHECK(do_interp(c));
+  if (n > 0) {
+    int r = dispatch_real(c, [&](auto *tag) -> int {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      PG_LAUNCH(c, "rk2_correct", nblocks(n), kBlock, k_rk2_correct<R>, n, c->sw.pos.as<double>(), c->sw.vel.as<R>(),
+                c->sw.status.as<int32_t>(), c->pos0.as<double>(), c->vel0.as<R>(), 0.5 * dt);
+      return PICGPU_OK;
+    });
+    PG_CHECK(r);
+  }
+  PG_CHECK(do_locate(c, 1));
+  return PICGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// stage-wise C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int picgpu_reset_statuses(picgpu_ctx *c) {
+  if (!c) return PICGPU_E_ARG;
+  PG_HIP(hipSetDevice(c->device));
+  return do_reset_advect(c, 0.0, 1, 0);
+}
+
+int picgpu_advect(picgpu_ctx *c, double dt) {
+  if (!c) return PICGPU_E_ARG;
+  PG_HIP(hipSetDevice(c->device));
+  return do_reset_advect(c, dt, 0, 1);
+}
+
+int picgpu_locate(picgpu_ctx *c) {
+  if (!c) return PICGPU_E_ARG;
+  PG_HIP(hipSetDevice(c->device));
+  return do_locate(c, 0);
+}
+
+int picgpu_remove_lost(picgpu_ctx *c, int64_t *removed) {
+  if (!c) return PICGPU_E_ARG;
+  PG_HIP(hipSetDevice(c->device));
+  int64_t rl = 0;
+  PG_CHECK(do_remove(c, 1, 0, &rl, nullptr));
+  c->stats.removed_lost = rl;
+  if (removed) *removed = rl;
+  return PICGPU_OK;
+}
+
+int picgpu_remove_out_of_bounds(picgpu_ctx *c, int64_t *removed) {
+  if (!c) return PICGPU_E_ARG;
+  PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(need_grid(c));
+  int64_t ro = 0;
+  PG_CHECK(do_remove(c, 0, 1, nullptr, &ro));
+  c->stats.removed_out_of_bounds = ro;
+  if (removed) *removed = ro;
+  return PICGPU_OK;
+}
+
+int picgpu_interpolate(picgpu_ctx *c) {
+  if (!c) return PICGPU_E_ARG;
+  PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(need_grid(c));
+  PG_CHECK(do_center_to_corner(c));
+  PG_CHECK(do_interp(c));
+  PG_HIP(hipStreamSynchronize(c->stream));
+  return PICGPU_OK;
+}
+
+int picgpu_scatter(picgpu_ctx *c) {
+  if (!c) return PICGPU_E_ARG;
+  PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(need_grid(c));
+  PG_CHECK(do_sort(c));
+  PG_CHECK(do_scatter_P(c));
+  PG_HIP(hipStreamSynchronize(c->stream));
+  return PICGPU_OK;
+}
+
+int picgpu_deposit_faces(picgpu_ctx *c, double particle_volume) {
+  if (!c) return PICGPU_E_ARG;
+  PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(need_grid(c));
+  if (!c->sorted) PG_CHECK(do_sort(c));
+  c->particle_volume = particle_volume;
+  PG_CHECK(do_deposit_faces(c));
+  PG_HIP(hipStreamSynchronize(c->stream));
+  return PICGPU_OK;
+}
+
+int picgpu_step(picgpu_ctx *c, double dt, int nsteps, unsigned flags) {
+  if (!c || nsteps < 0) return PICGPU_E_ARG;
+  PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(need_grid(c));
+  if (!c->have_ucat) {
+    set_error("ucat not uploaded");
+    return PICGPU_E_STATE;
+  }
+  for (int s = 0; s < nsteps; ++s) {
+    c->stats.particle_steps += c->n;
+    PG_CHECK(do_center_to_corner(c));
+    if (flags & PICGPU_STEP_RK2) {
+      PG_CHECK(do_rk2(c, dt));
+    } else {
+      PG_CHECK(do_reset_advect(c, dt, 1, 1));
+      PG_CHECK(do_locate(c, 0));
+    }
+    int64_t rl = 0, ro = 0;
+    PG_CHECK(do_remove(c, 1, 1, &rl, &ro));
+    c->stats.removed_lost = rl;
+    c->stats.removed_out_of_bounds = ro;
+    PG_CHECK(do_interp(c));
+    PG_CHECK(do_sort(c));
+    if (flags & PICGPU_STEP_DEPOSIT_CELLS) PG_CHECK(do_scatter_P(c));
+    if (flags & PICGPU_STEP_DEPOSIT_FACES) PG_CHECK(do_deposit_faces(c));
+  }
+  PG_HIP(hipStreamSynchronize(c->stream));
+  return PICGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// grid outputs
+// ---------------------------------------------------------------------------
+int picgpu_grid_field_size(picgpu_ctx *c, int which, int64_t *count) {
+  if (!c || !count) return PICGPU_E_ARG;
+  PG_CHECK(need_grid(c));
+  switch (which) {
+    case PICGPU_FIELD_PARTICLE_COUNT:
+    case PICGPU_FIELD_P_AVERAGE: *count = c->nnode; break;
+    case PICGPU_FIELD_CORNER_VELOCITY: *count = 3 * c->nnode; break;
+    case PICGPU_FIELD_IFACE:
+    case PICGPU_FIELD_JFACE:
+    case PICGPU_FIELD_KFACE: *count = 4 * nface(c, which - PICGPU_FIELD_IFACE); break;
+    case PICGPU_FIELD_IFACE_VOLFRAC:
+    case PICGPU_FIELD_JFACE_VOLFRAC:
+    case PICGPU_FIELD_KFACE_VOLFRAC: *count = nface(c, which - PICGPU_FIELD_IFACE_VOLFRAC); break;
+    default: set_error("unknown field %d", which); return PICGPU_E_ARG;
+  }
+  return PICGPU_OK;
+}
+
+int picgpu_download_grid_field(picgpu_ctx *c, int which, double *out) {
+  if (!c || !out) return PICGPU_E_ARG;
+  PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(need_grid(c));
+  PG_HIP(hipStreamSynchronize(c->stream));
+  const int64_t nx = c->IM + 1, ny = c->JM + 1;
+  if (which == PICGPU_FIELD_PARTICLE_COUNT) {
+    if (!c->sorted) {
+      set_error("ParticleCount needs picgpu_scatter/picgpu_deposit_faces/picgpu_step first");
+      return PICGPU_E_STATE;
+    }
+    std::vector<int32_t> h((size_t)c->ncell);
+    PG_HIP(hipMemcpy(h.data(), c->cell_count.ptr, sizeof(int32_t) * (size_t)c->ncell, hipMemcpyDeviceToHost));
+    memset(out, 0, sizeof(double) * (size_t)c->nnode);
+    for (int64_t k = 0; k < c->KM; ++k)
+      for (int64_t j = 0; j < c->JM; ++j)
+        for (int64_t i = 0; i < c->IM; ++i) out[(k * ny + j) * nx + i] = (double)h[(k * c->JM + j) * c->IM + i];
+    return PICGPU_OK;
+  }
+  return dispatch_real(c, [&](auto *tag) -> int {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    if (which == PICGPU_FIELD_P_AVERAGE) {
+      if (!c->have_pavg) {
+        set_error("P average not computed");
+        return PICGPU_E_STATE;
+      }
+      std::vector<R> h((size_t)c->ncell);
+      PG_HIP(hipMemcpy(h.data(), c->pavg.ptr, sizeof(R) * (size_t)c->ncell, hipMemcpyDeviceToHost));
+      memset(out, 0, sizeof(double) * (size_t)c->nnode);
+      for (int64_t k = 0; k < c->KM; ++k)
+        for (int64_t j = 0; j < c->JM; ++j)
+          for (int64_t i = 0; i < c->IM; ++i) out[(k * ny + j) * nx + i] = (double)h[(k * c->JM + j) * c->IM + i];
+      return PICGPU_OK;
+    }
+    if (which == PICGPU_FIELD_CORNER_VELOCITY) {
+      if (!c->have_ucat) return PICGPU_E_STATE;
+      std::vector<R> h((size_t)c->nnode * 3);
+      PG_HIP(hipMemcpy(h.data(), c->corner.ptr, sizeof(R) * h.size(), hipMemcpyDeviceToHost));
+      for (size_t q = 0; q < h.size(); ++q) out[q] = (double)h[q];
+      return PICGPU_OK;
+    }
+    if (which >= PICGPU_FIELD_IFACE && which <= PICGPU_FIELD_KFACE) {
+      if (!c->have_faces) {
+        set_error("faces not deposited");
+        return PICGPU_E_STATE;
+      }
+      int dir = which - PICGPU_FIELD_IFACE;
+      std::vector<R> h((size_t)nface(c, dir) * 4);
+      PG_HIP(hipMemcpy(h.data(), c->face[dir].ptr, sizeof(R) * h.size(), hipMemcpyDeviceToHost));
+      for (size_t q = 0; q < h.size(); ++q) out[q] = (double)h[q];
+      return PICGPU_OK;
+    }
+    if (which >= PICGPU_FIELD_IFACE_VOLFRAC && which <= PICGPU_FIELD_KFACE_VOLFRAC) {
+      if (!c->have_faces) {
+        set_error("faces not deposited");
+        return PICGPU_E_STATE;
+      }
+      int dir = which - PICGPU_FIELD_IFACE_VOLFRAC;
+      int64_t nf = nface(c, dir);
+      DevBuf tmp;
+      PG_CHECK(tmp.ensure(sizeof(double) * (size_t)nf));
+      {
+        TimerRec _rec;
+        timer_begin(c, "face_volfrac", _rec);
+        hipLaunchKernelGGL(k_face_volfrac<R>, dim3(nblocks(nf)), dim3(kBlock), 0, c->stream, dir, grid_view(c),
+                           c->face[dir].as<R>(), c->particle_volume, tmp.as<double>());
+        timer_end(c, _rec);
+      }
+      hipError_t e1 = hipGetLastError();
+      hipError_t e2 = hipMemcpyAsync(out, tmp.ptr, sizeof(double) * (size_t)nf, hipMemcpyDeviceToHost, c->stream);
+      hipError_t e3 = hipStreamSynchronize(c->stream);
+      tmp.release();
+      if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        set_error("face volume fraction failed");
+        return PICGPU_E_HIP;
+      }
+      return PICGPU_OK;
+    }
+    set_error("unknown field %d", which);
+    return PICGPU_E_ARG;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// statistics and timing
+// ---------------------------------------------------------------------------
+int picgpu_get_stats(picgpu_ctx *c, picgpu_stats *st) {
+  if (!c || !st) return PICGPU_E_ARG;
+  *st = c->stats;
+  st->n_particles = c->n;
+  return PICGPU_OK;
+}
+
+int picgpu_timing_enable(picgpu_ctx *c, int on) {
+  if (!c) return PICGPU_E_ARG;
+  PG_CHECK(flush_timers(c));
+  c->timing = on != 0;
+  return PICGPU_OK;
+}
+
+int picgpu_timing_query(picgpu_ctx *c, const char *kernel, double *total_ms, int64_t *launches) {
+  if (!c || !kernel) return PICGPU_E_ARG;
+  PG_CHECK(flush_timers(c));
+  auto it = c->timing_acc.find(kernel);
+  if (total_ms) *total_ms = it == c->timing_acc.end() ? 0.0 : it->second.first;
+  if (launches) *launches = it == c->timing_acc.end() ? 0 : it->second.second;
+  return PICGPU_OK;
+}
+
+int picgpu_timing_reset(picgpu_ctx *c) {
+  if (!c) return PICGPU_E_ARG;
+  PG_CHECK(flush_timers(c));
+  c->timing_acc.clear();
+  return PICGPU_OK;
+}
+
+}  // extern "C"

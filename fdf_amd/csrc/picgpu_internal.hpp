@@ -1,0 +1,179 @@
+// picgpu_internal.hpp -- context and shared helpers of the gfx950 particle loop.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/picgpu.h"
+#include "walk.hpp"
+
+namespace picgpu {
+
+void set_error(const char *fmt, ...);
+
+#define PG_HIP(call)                                                                        \
+  do {                                                                                      \
+    hipError_t _e = (call);                                                                 \
+    if (_e != hipSuccess) {                                                                 \
+      ::picgpu::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), __FILE__, __LINE__); \
+      return PICGPU_E_HIP;                                                                  \
+    }                                                                                       \
+  } while (0)
+
+#define PG_CHECK(expr)          \
+  do {                          \
+    int _r = (expr);            \
+    if (_r != PICGPU_OK) return _r; \
+  } while (0)
+
+// A device buffer that grows on demand.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want) {
+    if (want <= bytes && ptr) return PICGPU_OK;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    if (want == 0) want = 256;
+    if (hipMalloc(&ptr, want) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("hipMalloc(%zu) failed", want);
+      return PICGPU_E_NOMEM;
+    }
+    bytes = want;
+    return PICGPU_OK;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T *as() const {
+    return static_cast<T *>(ptr);
+  }
+};
+
+// Particle arrays (structure of arrays).  vel, w and P hold `real` elements:
+// double in PICGPU_FP64 mode, float in PICGPU_FP32 mode.
+struct SwarmBufs {
+  DevBuf pos;     // double [n][3]
+  DevBuf vel;     // real [n][3]
+  DevBuf w;       // real [n][3]
+  DevBuf cell;    // int32 [n], packed i + IM*(j + JM*k), -1 = none
+  DevBuf status;  // int32 [n]
+  DevBuf pid;     // int64 [n]
+  DevBuf P;       // real [n]
+};
+
+enum Counter : int {
+  C_WALK_ITERS = 0,
+  C_WALK_MAX = 1,
+  C_FALLBACKS = 2,
+  C_LOCATED = 3,
+  C_LOST = 4,
+  C_ERROR = 5,
+  C_KEEP = 6,
+  C_REMOVED_LOST = 7,
+  C_REMOVED_OOB = 8,
+  C_NUM = 16
+};
+
+struct TimerRec {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+}  // namespace picgpu
+
+struct picgpu_ctx {
+  picgpu_options opt;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  size_t rsize = 8;  // sizeof(real)
+
+  // grid
+  bool have_grid = false;
+  int IM = 0, JM = 0, KM = 0;
+  int64_t ncell = 0, nnode = 0;
+  picgpu::DevBuf coor;  // double [nnode][3]
+  double bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+
+  // Eulerian fields
+  bool have_ucat = false;
+  picgpu::DevBuf ucat;    // real [ncell][3] (compact, cell-linear)
+  picgpu::DevBuf corner;  // real [nnode][3]
+
+  // swarm (double-buffered for sorting / compaction)
+  int64_t n = 0;
+  picgpu::SwarmBufs sw, alt;
+  picgpu::DevBuf rank;      // int32 [n] slot within the cell
+  picgpu::DevBuf keep;      // int32 [n+1] keep flags / compaction offsets
+  picgpu::DevBuf walk_steps;// int32 [n]
+  picgpu::DevBuf guess;     // int32 [n] MLP start cell
+  picgpu::DevBuf pos0, vel0;// RK2 saved state
+  bool sorted = false;      // particles sorted by cell, cell_start valid
+
+  // per-cell / per-face outputs
+  picgpu::DevBuf cell_count;  // int32 [ncell+1]
+  picgpu::DevBuf cell_start;  // int32 [ncell+1]
+  picgpu::DevBuf pavg;        // real [ncell]
+  picgpu::DevBuf face[3];     // real [nface_d][4]
+  bool have_pavg = false, have_faces = false;
+  double particle_volume = 1.0;
+
+  // scratch for scans
+  picgpu::DevBuf scan_tmp;
+
+  // counters
+  picgpu::DevBuf counters;  // uint64 [C_NUM]
+  picgpu_stats stats{};
+
+  // timing
+  bool timing = false;
+  std::vector<picgpu::TimerRec> pending;
+  std::vector<hipEvent_t> event_pool;
+  std::map<std::string, std::pair<double, int64_t>> timing_acc;
+
+  // MLP
+  picgpu::DevBuf mlp_master;  // float [P]
+  picgpu::DevBuf mlp_bf16;    // packed bf16 weights for kernels
+  picgpu::DevBuf mlp_adam_m, mlp_adam_v, mlp_grad, mlp_partial;
+  picgpu::DevBuf mlp_x, mlp_t, mlp_y;
+  bool have_mlp = false;
+  double mlp_lo[3] = {0, 0, 0}, mlp_scale[3] = {1, 1, 1};
+};
+
+namespace picgpu {
+
+// kernel timing around a launch
+hipEvent_t take_event(picgpu_ctx *c);
+void timer_begin(picgpu_ctx *c, const char *name, TimerRec &rec);
+void timer_end(picgpu_ctx *c, TimerRec &rec);
+int flush_timers(picgpu_ctx *c);
+
+// device exclusive scan of int32 (n entries -> out[0..n], out[n] = total)
+int scan_exclusive_i32(picgpu_ctx *c, const int32_t *in, int32_t *out, int64_t n);
+
+// MLP
+int mlp_guess_device(picgpu_ctx *c, int64_t n, const double *d_pos, const int32_t *d_cell_or_null,
+                     int32_t *d_guess);
+
+}  // namespace picgpu
+
+#define PG_LAUNCH(ctx, name, grid, block, kernel, ...)                                    \
+  do {                                                                                    \
+    ::picgpu::TimerRec _rec;                                                              \
+    ::picgpu::timer_begin((ctx), (name), _rec);                                           \
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, (ctx)->stream, __VA_ARGS__);   \
+    hipError_t _le = hipGetLastError();                                                   \
+    if (_le != hipSuccess) {                                                              \
+      ::picgpu::set_error("launch %s failed: %s", (name), hipGetErrorString(_le));        \
+      return PICGPU_E_HIP;                                                                \
+    }                                                                                     \
+    ::picgpu::timer_end((ctx), _rec);                                                     \
+  } while (0)

@@ -1,0 +1,176 @@
+"""Host-side mirror of the reference's per-step particle interface, over the C-ABI.
+
+Method names follow the reference entry points they stand in for (one MPI
+rank); errors raise PicgpuError (the reference returns a nonzero
+PetscErrorCode); per-particle outcomes are statuses.
+
+    ResetAllParticleStatuses          ParticleMotion.c:2149-2176
+    UpdateAllParticlePositions        ParticleMotion.c:38-95
+    LocateAllParticlesInGrid          ParticleMotion.c:1931-2135 (walk: walkingsearch.c:1094-1242)
+    CheckAndRemoveLostParticles       ParticleMotion.c:532-625
+    CheckAndRemoveOutOfBoundsParticles ParticleMotion.c:283-383
+    InterpolateAllFieldsToSwarm       interpolation.c:1660-1711
+    ScatterAllParticleFieldsToEulerFields interpolation.c:2223-2313
+    AdvanceSimulation                 the particle stages of AdvanceSimulation_TEST simulation.c:248-292
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class PicGpu:
+    """One GPU context: grid, Eulerian field and swarm resident in HBM."""
+
+    def __init__(self, precision: str = "fp64", use_mlp_guess: bool = False, max_traversal: int = 1000,
+                 collect_walk_stats: bool = False, device: int = 0):
+        self.lib = L.load()
+        opt = L.PicgpuOptions()
+        self.lib.picgpu_default_options(C.byref(opt))
+        opt.precision = {"fp64": L.FP64, "fp32": L.FP32}[precision]
+        opt.use_mlp_guess = int(use_mlp_guess)
+        opt.max_traversal = int(max_traversal)
+        opt.collect_walk_stats = int(collect_walk_stats)
+        opt.device = int(device)
+        self.precision = precision
+        h = C.c_void_p()
+        L.check(self.lib.picgpu_create(C.byref(h), C.byref(opt)))
+        self.h = h
+        self.IM = self.JM = self.KM = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.picgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- inputs ----
+    def upload_grid(self, IM: int, JM: int, KM: int, coor: np.ndarray):
+        coor = np.ascontiguousarray(coor, dtype=np.float64)
+        assert coor.size == 3 * (IM + 1) * (JM + 1) * (KM + 1)
+        L.check(self.lib.picgpu_upload_grid(self.h, IM, JM, KM, _ptr(coor)))
+        self.IM, self.JM, self.KM = IM, JM, KM
+
+    def upload_ucat(self, ucat: np.ndarray):
+        ucat = np.ascontiguousarray(ucat, dtype=np.float64)
+        assert ucat.size == 3 * (self.IM + 1) * (self.JM + 1) * (self.KM + 1)
+        L.check(self.lib.picgpu_upload_ucat(self.h, _ptr(ucat)))
+
+    def upload_particles(self, pos, vel=None, cell=None, weight=None, status=None, pid=None, P=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        n = pos.shape[0]
+        cell = (np.full((n, 3), -1, np.int64) if cell is None else np.ascontiguousarray(cell, np.int64).reshape(n, 3))
+        status = (np.full(n, L.UNINITIALIZED, np.int64) if status is None else np.ascontiguousarray(status, np.int64))
+        pid = np.arange(n, dtype=np.int64) if pid is None else np.ascontiguousarray(pid, np.int64)
+        vel = None if vel is None else np.ascontiguousarray(vel, np.float64).reshape(n, 3)
+        weight = None if weight is None else np.ascontiguousarray(weight, np.float64).reshape(n, 3)
+        P = None if P is None else np.ascontiguousarray(P, np.float64)
+        L.check(self.lib.picgpu_upload_particles(self.h, n, _ptr(pos), _ptr(vel), _ptr(cell), _ptr(weight),
+                                                  _ptr(status), _ptr(pid), _ptr(P)))
+
+    # ---- outputs ----
+    @property
+    def n(self) -> int:
+        v = C.c_int64()
+        L.check(self.lib.picgpu_num_particles(self.h, C.byref(v)))
+        return v.value
+
+    def download_particles(self) -> dict:
+        n = self.n
+        out = {
+            "pos": np.empty((n, 3), np.float64),
+            "vel": np.empty((n, 3), np.float64),
+            "cell": np.empty((n, 3), np.int64),
+            "weight": np.empty((n, 3), np.float64),
+            "status": np.empty(n, np.int64),
+            "pid": np.empty(n, np.int64),
+        }
+        L.check(self.lib.picgpu_download_particles(self.h, _ptr(out["pos"]), _ptr(out["vel"]), _ptr(out["cell"]),
+                                                    _ptr(out["weight"]), _ptr(out["status"]), _ptr(out["pid"])))
+        return out
+
+    def walk_steps(self) -> np.ndarray:
+        s = np.empty(self.n, np.int32)
+        L.check(self.lib.picgpu_download_walk_steps(self.h, _ptr(s)))
+        return s
+
+    def field(self, which: int) -> np.ndarray:
+        cnt = C.c_int64()
+        L.check(self.lib.picgpu_grid_field_size(self.h, which, C.byref(cnt)))
+        out = np.empty(cnt.value, np.float64)
+        L.check(self.lib.picgpu_download_grid_field(self.h, which, _ptr(out)))
+        return out
+
+    def stats(self) -> dict:
+        st = L.PicgpuStats()
+        L.check(self.lib.picgpu_get_stats(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in L.PicgpuStats._fields_}
+
+    # ---- reference entry points ----
+    def ResetAllParticleStatuses(self):
+        L.check(self.lib.picgpu_reset_statuses(self.h))
+
+    def UpdateAllParticlePositions(self, dt: float):
+        L.check(self.lib.picgpu_advect(self.h, dt))
+
+    def LocateAllParticlesInGrid(self):
+        L.check(self.lib.picgpu_locate(self.h))
+
+    def CheckAndRemoveLostParticles(self) -> int:
+        r = C.c_int64()
+        L.check(self.lib.picgpu_remove_lost(self.h, C.byref(r)))
+        return r.value
+
+    def CheckAndRemoveOutOfBoundsParticles(self) -> int:
+        r = C.c_int64()
+        L.check(self.lib.picgpu_remove_out_of_bounds(self.h, C.byref(r)))
+        return r.value
+
+    def InterpolateAllFieldsToSwarm(self):
+        L.check(self.lib.picgpu_interpolate(self.h))
+
+    def ScatterAllParticleFieldsToEulerFields(self):
+        L.check(self.lib.picgpu_scatter(self.h))
+
+    def DepositParticlesToFaces(self, particle_volume: float = 1.0):
+        L.check(self.lib.picgpu_deposit_faces(self.h, particle_volume))
+
+    def AdvanceSimulation(self, dt: float, nsteps: int, rk2: bool = False, deposit_cells: bool = True,
+                          deposit_faces: bool = False):
+        flags = (L.STEP_RK2 if rk2 else 0) | (L.STEP_DEPOSIT_CELLS if deposit_cells else 0) | \
+                (L.STEP_DEPOSIT_FACES if deposit_faces else 0)
+        L.check(self.lib.picgpu_step(self.h, dt, nsteps, flags))
+
+    def synchronize(self):
+        L.check(self.lib.picgpu_synchronize(self.h))
+
+    # ---- timing ----
+    def timing(self, on: bool = True):
+        L.check(self.lib.picgpu_timing_enable(self.h, int(on)))
+
+    def timing_reset(self):
+        L.check(self.lib.picgpu_timing_reset(self.h))
+
+    def timing_query(self, kernel: str):
+        ms = C.c_double()
+        cnt = C.c_int64()
+        L.check(self.lib.picgpu_timing_query(self.h, kernel.encode(), C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value

@@ -1,0 +1,174 @@
+/*
+ * picgpu.h -- C-ABI of the MI355X (gfx950) particle loop for PICurv.
+ *
+ * This is the drop-in boundary for the reference's per-step particle loop.
+ * Plain C types only; the context owns device memory, the caller owns host
+ * buffers.  Every call is synchronous at return unless stated otherwise.
+ * One host thread per context.  Return value: PICGPU_OK (0) or a negative
+ * PICGPU_E_* code; picgpu_last_error() gives a message.  Per-particle
+ * outcomes are statuses with the reference's enum values
+ * (common.h:75-124), never errors.
+ *
+ * Reference entry points each call replaces (one MPI rank):
+ *   picgpu_reset_statuses     <- ResetAllParticleStatuses        ParticleMotion.c:2149-2176
+ *   picgpu_advect             <- UpdateAllParticlePositions      ParticleMotion.c:38-95
+ *   picgpu_locate             <- LocateAllParticlesInGrid_TEST   ParticleMotion.c:1931-2135
+ *                                 (walk: LocateParticleOrFindMigrationTarget_TEST
+ *                                  walkingsearch.c:1094-1242)
+ *   picgpu_remove_lost        <- CheckAndRemoveLostParticles     ParticleMotion.c:532-625
+ *   picgpu_remove_out_of_bounds <- CheckAndRemoveOutOfBoundsParticles ParticleMotion.c:283-383
+ *   picgpu_interpolate        <- InterpolateAllFieldsToSwarm     interpolation.c:1660-1711
+ *   picgpu_scatter            <- ScatterAllParticleFieldsToEulerFields interpolation.c:2223-2313
+ *                                 (ParticleCount: CalculateParticleCountPerCell
+ *                                  ParticleMotion.c:931-1067)
+ *   picgpu_step               <- the particle stages of AdvanceSimulation_TEST simulation.c:248-292
+ * Additions with no reference counterpart: picgpu_deposit_faces (face
+ * deposition), the RK2 option of picgpu_step, the FP32 precision mode and
+ * the MLP initial guess (picgpu_mlp_*).
+ */
+#ifndef PICGPU_H
+#define PICGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ---- */
+#define PICGPU_OK 0
+#define PICGPU_E_ARG (-1)        /* bad argument (PETSC_ERR_ARG_*) */
+#define PICGPU_E_STATE (-2)      /* call out of order (PETSC_ERR_ARG_WRONGSTATE) */
+#define PICGPU_E_DEGENERATE (-3) /* |n| < 1e-12 on a face, walkingsearch.c:147-155 (PETSC_ERR_USER) */
+#define PICGPU_E_HIP (-4)        /* HIP runtime error (PETSC_ERR_LIB) */
+#define PICGPU_E_NOMEM (-5)      /* device allocation failed (PETSC_ERR_MEM) */
+#define PICGPU_E_UNSUPPORTED (-6)/* (PETSC_ERR_SUP) */
+
+/* ---- particle status (common.h:75-124) ---- */
+#define PICGPU_NEEDS_LOCATION 0
+#define PICGPU_ACTIVE_AND_LOCATED 1
+#define PICGPU_MIGRATING_OUT 2
+#define PICGPU_LOST 3
+#define PICGPU_UNINITIALIZED 4
+
+/* ---- precision modes ---- */
+#define PICGPU_FP64 0 /* plain: reference arithmetic, bit-exact cells/status/weights */
+#define PICGPU_FP32 1 /* FP32 weights/interp/advect/deposit; FP64 positions and near-face tests */
+
+/* ---- grid fields for picgpu_download_grid_field ---- */
+#define PICGPU_FIELD_PARTICLE_COUNT 0 /* ParticleCount, node slots [(KM+1)(JM+1)(IM+1)] */
+#define PICGPU_FIELD_P_AVERAGE 1      /* P scatter result, node slots */
+#define PICGPU_FIELD_CORNER_VELOCITY 2/* centre->corner ucat, node slots x3 */
+#define PICGPU_FIELD_IFACE 3          /* [KM][JM][IM+1] x4 (count, mx, my, mz) */
+#define PICGPU_FIELD_JFACE 4          /* [KM][JM+1][IM] x4 */
+#define PICGPU_FIELD_KFACE 5          /* [KM+1][JM][IM] x4 */
+#define PICGPU_FIELD_IFACE_VOLFRAC 6  /* [KM][JM][IM+1]: count * V_p / V_dual */
+#define PICGPU_FIELD_JFACE_VOLFRAC 7
+#define PICGPU_FIELD_KFACE_VOLFRAC 8
+
+/* ---- picgpu_step flags ---- */
+#define PICGPU_STEP_RK2 0x1           /* Heun predictor-corrector instead of forward Euler */
+#define PICGPU_STEP_DEPOSIT_CELLS 0x2 /* ParticleCount + P average */
+#define PICGPU_STEP_DEPOSIT_FACES 0x4 /* face count + momentum */
+
+typedef struct picgpu_ctx picgpu_ctx;
+
+typedef struct {
+  int precision;            /* PICGPU_FP64 or PICGPU_FP32 */
+  int use_mlp_guess;        /* 1: MLP start cell for particles with no valid cell */
+  int max_traversal;        /* walk cap, reference MAX_TRAVERSAL = 1000 (walkingsearch.c:10) */
+  double distance_threshold;/* 1e-11 (walkingsearch.c:11) */
+  double weight_clamp;      /* 1e-14 (ParticleSwarm.c:5) */
+  double bbox_tolerance;    /* 1e-6 (grid.c:5) */
+  int device;               /* HIP device ordinal */
+  int collect_walk_stats;   /* 1: record per-particle walk iterations of the last locate */
+} picgpu_options;
+
+typedef struct {
+  int64_t n_particles;      /* current swarm size */
+  int64_t located;          /* ACTIVE_AND_LOCATED after the last locate */
+  int64_t lost;             /* LOST after the last locate */
+  int64_t walk_iterations;  /* sum of walk iterations in the last locate */
+  int64_t walk_max;         /* max walk iterations of one particle in the last locate */
+  int64_t fp64_fallbacks;   /* FP32 mode: faces re-evaluated in FP64 in the last locate */
+  int64_t removed_lost;     /* by the last remove calls */
+  int64_t removed_out_of_bounds;
+  int64_t particle_steps;   /* particles processed, summed over picgpu_step steps */
+} picgpu_stats;
+
+/* ---- lifetime ---- */
+void picgpu_default_options(picgpu_options *opt);
+int picgpu_create(picgpu_ctx **ctx, const picgpu_options *opt);
+int picgpu_destroy(picgpu_ctx *ctx);
+const char *picgpu_last_error(void);
+int picgpu_synchronize(picgpu_ctx *ctx);
+
+/* ---- grid and Eulerian input ---- */
+/* coor: [(KM+1)(JM+1)(IM+1)][3] doubles, k slowest (the fda coordinate Vec, grid.c:386-395) */
+int picgpu_upload_grid(picgpu_ctx *ctx, int64_t IM, int64_t JM, int64_t KM, const double *coor);
+/* ucat: node-slot layout [(KM+1)(JM+1)(IM+1)][3]; cell (i,j,k), i<IM, j<JM, k<KM,
+ * is stored at slot (i,j,k) (README.md:242-260); other slots are ignored. */
+int picgpu_upload_ucat(picgpu_ctx *ctx, const double *ucat);
+
+/* ---- swarm I/O (DMSwarm field layout, ParticleSwarm.c:64-96) ---- */
+/* pos, vel, weight: [n][3]; cell: [n][3] int64 (-1 = none); status: [n] int64;
+ * pid: [n] int64; P: [n] (NULL -> 101325.0, ParticleSwarm.c:514-518).
+ * vel, weight may be NULL (-> 0). */
+int picgpu_upload_particles(picgpu_ctx *ctx, int64_t n, const double *pos, const double *vel,
+                            const int64_t *cell, const double *weight, const int64_t *status,
+                            const int64_t *pid, const double *P);
+int picgpu_num_particles(picgpu_ctx *ctx, int64_t *n);
+/* any output pointer may be NULL; arrays must hold the current n particles */
+int picgpu_download_particles(picgpu_ctx *ctx, double *pos, double *vel, int64_t *cell,
+                              double *weight, int64_t *status, int64_t *pid);
+/* per-particle walk iterations of the last locate (collect_walk_stats=1) */
+int picgpu_download_walk_steps(picgpu_ctx *ctx, int32_t *steps);
+
+/* ---- stage-wise entry points (reference names in the header comment) ---- */
+int picgpu_reset_statuses(picgpu_ctx *ctx);
+int picgpu_advect(picgpu_ctx *ctx, double dt);
+int picgpu_locate(picgpu_ctx *ctx);
+int picgpu_remove_lost(picgpu_ctx *ctx, int64_t *removed);
+int picgpu_remove_out_of_bounds(picgpu_ctx *ctx, int64_t *removed);
+int picgpu_interpolate(picgpu_ctx *ctx);
+int picgpu_scatter(picgpu_ctx *ctx);
+int picgpu_deposit_faces(picgpu_ctx *ctx, double particle_volume);
+
+/* ---- whole steps: reset, advect(+locate) [Euler|RK2], remove, interpolate, deposit ---- */
+int picgpu_step(picgpu_ctx *ctx, double dt, int nsteps, unsigned flags);
+
+/* ---- grid outputs ---- */
+int picgpu_grid_field_size(picgpu_ctx *ctx, int which, int64_t *count);
+int picgpu_download_grid_field(picgpu_ctx *ctx, int which, double *out);
+
+/* ---- statistics and timing ---- */
+int picgpu_get_stats(picgpu_ctx *ctx, picgpu_stats *st);
+/* Kernel timing with HIP events on the library's stream: enable, then read the
+ * summed milliseconds and launch count per kernel name. */
+int picgpu_timing_enable(picgpu_ctx *ctx, int on);
+int picgpu_timing_query(picgpu_ctx *ctx, const char *kernel, double *total_ms, int64_t *launches);
+int picgpu_timing_reset(picgpu_ctx *ctx);
+
+/* ---- MLP initial guess (3 -> 64 -> 64 -> 3, GELU, BF16 weights, FP32 accumulation) ---- */
+#define PICGPU_MLP_HIDDEN 64
+/* number of FP32 parameters: W1[64][3] b1[64] W2[64][64] b2[64] W3[3][64] b3[3] */
+int64_t picgpu_mlp_param_count(void);
+int picgpu_mlp_set_params(picgpu_ctx *ctx, const float *params);
+int picgpu_mlp_get_params(picgpu_ctx *ctx, float *params);
+/* Train on (position, located cell + weights) pairs of the current swarm
+ * (located particles only), Adam, minibatch `batch`, `iters` iterations. */
+int picgpu_mlp_train(picgpu_ctx *ctx, int64_t max_pairs, int batch, int iters, float lr,
+                     uint64_t seed, float *final_loss);
+/* Device forward/backward for testing: x [n][3] normalised inputs, t [n][3]
+ * targets; outputs y [n][3], loss and grad (FP32, param layout). */
+int picgpu_mlp_forward(picgpu_ctx *ctx, int64_t n, const float *x, float *y);
+int picgpu_mlp_loss_grad(picgpu_ctx *ctx, int64_t n, const float *x, const float *t, float *loss,
+                         float *grad);
+/* start cells the MLP proposes for positions pos [n][3]: cell [n][3] */
+int picgpu_mlp_guess(picgpu_ctx *ctx, int64_t n, const double *pos, int64_t *cell);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PICGPU_H */

@@ -1,0 +1,197 @@
+"""ctypes wrapper of the CPU oracle (oracle/libpicref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product path never uses it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libpicref.so")
+LIB_FAST = os.path.join(HERE, "libpicref_fast.so")
+
+LOCATED, LOST = 1, 3
+MAX_TRAVERSAL = 1000
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+class _Ctx(C.Structure):
+    _fields_ = [("IM", C.c_int64), ("JM", C.c_int64), ("KM", C.c_int64), ("coor", C.c_void_p),
+                ("corner", C.c_void_p), ("max_traversal", C.c_int)]
+
+
+class _Grid(C.Structure):
+    _fields_ = [("IM", C.c_int64), ("JM", C.c_int64), ("KM", C.c_int64), ("coor", C.c_void_p)]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    """Thin numpy front end over the C restatement."""
+
+    def __init__(self, fast: bool = False):
+        path = LIB_FAST if fast else LIB
+        if not os.path.exists(path):
+            build()
+        self.lib = C.CDLL(path)
+        lib = self.lib
+        P, I64, D, I = C.c_void_p, C.c_int64, C.c_double, C.c_int
+        sig = {
+            "ref_cell_char_size": (D, [P]),
+            "ref_face_distances": (I, [P, P, D, P]),
+            "ref_point_position": (I, [P]),
+            "ref_evaluate_position": (I, [P, P, P, P]),
+            "ref_update_weights": (None, [P, P]),
+            "ref_locate_all": (I, [C.POINTER(_Grid), I64, P, P, P, P, I, P, P]),
+            "ref_center_to_corner": (None, [I64, I64, I64, P, P]),
+            "ref_interp_all": (None, [I64, I64, I64, P, I64, P, P, P]),
+            "ref_advect_euler": (None, [I64, P, P, D]),
+            "ref_reset_statuses": (None, [I64, P]),
+            "ref_bbox": (None, [C.POINTER(_Grid), P, P]),
+            "ref_remove": (I64, [I64, P, P, P, P, P, P, P, P, P, P, P]),
+            "ref_count_cells": (None, [I64, I64, I64, I64, P, P]),
+            "ref_scatter_P": (None, [I64, I64, I64, I64, P, P, P, P, P]),
+            "ref_deposit_faces": (None, [I64, I64, I64, I64, P, P, P, P, P, P, P]),
+            "ref_step_euler": (I64, [C.POINTER(_Ctx), I64, P, P, P, P, P, P, P, D, P]),
+            "ref_step_rk2": (I64, [C.POINTER(_Ctx), I64, P, P, P, P, P, P, P, D, P]),
+            "ref_mlp_param_count": (I64, [I]),
+            "ref_mlp_forward": (None, [I, P, I64, P, P, I]),
+            "ref_mlp_loss_grad": (D, [I, P, I64, P, P, P, I]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+
+    # ---- primitives ----
+    def char_size(self, V):
+        V = np.ascontiguousarray(V, np.float64)
+        return self.lib.ref_cell_char_size(_p(V))
+
+    def evaluate(self, V, p):
+        V = np.ascontiguousarray(V, np.float64)
+        p = np.ascontiguousarray(p, np.float64)
+        d = np.zeros(6)
+        pos = C.c_int()
+        r = self.lib.ref_evaluate_position(_p(V), _p(p), _p(d), C.byref(pos))
+        return r, d, pos.value
+
+    def update_weights(self, d):
+        d = np.array(d, np.float64)
+        w = np.zeros(3)
+        self.lib.ref_update_weights(_p(d), _p(w))
+        return w
+
+    # ---- whole-array stages ----
+    def locate(self, IM, JM, KM, coor, pos, cell=None, weight=None, status=None, guess=None,
+               max_traversal=MAX_TRAVERSAL):
+        coor = np.ascontiguousarray(coor, np.float64)
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        n = pos.shape[0]
+        cell = np.full((n, 3), -1, np.int64) if cell is None else np.array(cell, np.int64).reshape(n, 3)
+        weight = np.zeros((n, 3)) if weight is None else np.array(weight, np.float64).reshape(n, 3)
+        status = np.full(n, 4, np.int64) if status is None else np.array(status, np.int64)
+        guess = None if guess is None else np.ascontiguousarray(guess, np.int64).reshape(n, 3)
+        steps = np.zeros(n, np.int32)
+        g = _Grid(IM, JM, KM, _p(coor))
+        r = self.lib.ref_locate_all(C.byref(g), n, _p(pos), _p(cell), _p(weight), _p(status), max_traversal,
+                                    _p(guess), _p(steps))
+        if r != 0:
+            raise RuntimeError(f"oracle locate error {r}")
+        return cell, weight, status, steps
+
+    def center_to_corner(self, IM, JM, KM, ucat):
+        ucat = np.ascontiguousarray(ucat, np.float64)
+        out = np.zeros_like(ucat)
+        self.lib.ref_center_to_corner(IM, JM, KM, _p(ucat), _p(out))
+        return out
+
+    def interp(self, IM, JM, KM, corner, cell, weight):
+        corner = np.ascontiguousarray(corner, np.float64)
+        cell = np.ascontiguousarray(cell, np.int64)
+        weight = np.ascontiguousarray(weight, np.float64)
+        n = cell.shape[0]
+        vel = np.zeros((n, 3))
+        self.lib.ref_interp_all(IM, JM, KM, _p(corner), n, _p(cell), _p(weight), _p(vel))
+        return vel
+
+    def bbox(self, IM, JM, KM, coor):
+        coor = np.ascontiguousarray(coor, np.float64)
+        g = _Grid(IM, JM, KM, _p(coor))
+        lo, hi = np.zeros(3), np.zeros(3)
+        self.lib.ref_bbox(C.byref(g), _p(lo), _p(hi))
+        return lo, hi
+
+    def count_cells(self, IM, JM, KM, cell):
+        cell = np.ascontiguousarray(cell, np.int64)
+        out = np.zeros((IM + 1) * (JM + 1) * (KM + 1))
+        self.lib.ref_count_cells(IM, JM, KM, cell.shape[0], _p(cell), _p(out))
+        return out
+
+    def scatter_P(self, IM, JM, KM, cell, P, pid, count):
+        cell = np.ascontiguousarray(cell, np.int64)
+        P = np.ascontiguousarray(P, np.float64)
+        order = np.ascontiguousarray(np.argsort(pid, kind="stable"), np.int64)
+        count = np.ascontiguousarray(count, np.float64)
+        out = np.zeros_like(count)
+        self.lib.ref_scatter_P(IM, JM, KM, cell.shape[0], _p(cell), _p(P), _p(order), _p(count), _p(out))
+        return out
+
+    def deposit_faces(self, IM, JM, KM, cell, weight, vel, pid):
+        cell = np.ascontiguousarray(cell, np.int64)
+        weight = np.ascontiguousarray(weight, np.float64)
+        vel = np.ascontiguousarray(vel, np.float64)
+        pid = np.ascontiguousarray(pid, np.int64)
+        fi = np.zeros(4 * KM * JM * (IM + 1))
+        fj = np.zeros(4 * KM * (JM + 1) * IM)
+        fk = np.zeros(4 * (KM + 1) * JM * IM)
+        self.lib.ref_deposit_faces(IM, JM, KM, cell.shape[0], _p(cell), _p(weight), _p(vel), _p(pid), _p(fi),
+                                   _p(fj), _p(fk))
+        return fi, fj, fk
+
+    def step(self, IM, JM, KM, coor, corner, state: dict, dt: float, rk2: bool = False,
+             max_traversal=MAX_TRAVERSAL):
+        """One reference step on a state dict (pos, vel, cell, weight, status, pid, P); compacts in place."""
+        coor = np.ascontiguousarray(coor, np.float64)
+        corner = np.ascontiguousarray(corner, np.float64)
+        s = state
+        for k, t in (("pos", np.float64), ("vel", np.float64), ("weight", np.float64), ("P", np.float64),
+                     ("cell", np.int64), ("status", np.int64), ("pid", np.int64)):
+            s[k] = np.ascontiguousarray(s[k], t)
+        n = s["pos"].shape[0]
+        ctx = _Ctx(IM, JM, KM, _p(coor), _p(corner), max_traversal)
+        removed = C.c_int64()
+        fn = self.lib.ref_step_rk2 if rk2 else self.lib.ref_step_euler
+        m = fn(C.byref(ctx), n, _p(s["pos"]), _p(s["vel"]), _p(s["cell"]), _p(s["weight"]), _p(s["status"]),
+               _p(s["pid"]), _p(s["P"]), dt, C.byref(removed))
+        if m < 0:
+            raise RuntimeError(f"oracle step error {m}")
+        for k in s:
+            s[k] = s[k][:m]
+        return removed.value
+
+    # ---- MLP (spec) ----
+    def mlp_forward(self, params, x, round_bf16=True, H=64):
+        params = np.ascontiguousarray(params, np.float64)
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        y = np.zeros_like(x)
+        self.lib.ref_mlp_forward(H, _p(params), x.shape[0], _p(x), _p(y), int(round_bf16))
+        return y
+
+    def mlp_loss_grad(self, params, x, t, round_bf16=True, H=64):
+        params = np.ascontiguousarray(params, np.float64)
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(t, np.float64).reshape(-1, 3)
+        g = np.zeros_like(params)
+        loss = self.lib.ref_mlp_loss_grad(H, _p(params), x.shape[0], _p(x), _p(t), _p(g), int(round_bf16))
+        return loss, g

@@ -1,0 +1,289 @@
+"""GPU parity: the gfx950 path through the C-ABI against the CPU oracle.
+
+Bar: owning cells, statuses and walk iterations bit-exact in both precision
+modes; FP64 mode weights, velocities, positions and deposits bit-exact; FP32
+mode weights/velocities/deposits within 1e-5 (relative to the field scale)
+of the FP64 oracle.  Particles are compared per PID (the device keeps them
+sorted by cell).
+"""
+import numpy as np
+import pytest
+
+from fdf_amd import _lib as L
+from fdf_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+FP32_TOL = 1e-5
+
+
+def _say(*a):
+    print("[gpu-case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def swarm_cls():
+    from fdf_amd.swarm import PicGpu
+
+    return PicGpu
+
+
+def _by_pid(d):
+    o = np.argsort(d["pid"])
+    return {k: v[o] for k, v in d.items()}
+
+
+GRIDS = {
+    "cube": lambda: (20, 20, 280, W.uniform_grid(20, 20, 280)),
+    "warped": lambda: (16, 16, 32, W.perturbed_cube(16, 16, 32, seed=11)),
+    "warped_strong": lambda: (8, 8, 12, W.perturbed_cube(8, 8, 12, seed=12, amp=0.35)),
+    "tube": lambda: (16, 16, 48, W.tube_grid(16, 16, 48)),
+}
+
+
+def _points(name, n, seed):
+    if name == "tube":
+        return W.particles_uniform_tube(n, seed)
+    p = np.random.default_rng(seed).random((n, 3))
+    p[: n // 50] = p[: n // 50] * 1.4 - 0.2  # some outside the domain
+    return p
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("grid", list(GRIDS))
+def test_locate_from_scratch(swarm_cls, oracle, grid, precision):
+    _say("locate_from_scratch", grid, precision)
+    IM, JM, KM, coor = GRIDS[grid]()
+    pos = _points(grid, 20000, 7)
+    cell, w, st, steps = oracle.locate(IM, JM, KM, coor, pos)
+    with swarm_cls(precision=precision, collect_walk_stats=True) as g:
+        g.upload_grid(IM, JM, KM, coor)
+        g.upload_particles(pos)
+        g.LocateAllParticlesInGrid()
+        gsteps = g.walk_steps()
+        d = g.download_particles()
+        s = g.stats()
+    o = np.argsort(d["pid"])
+    np.testing.assert_array_equal(d["status"][o], st)
+    np.testing.assert_array_equal(d["cell"][o], cell)
+    np.testing.assert_array_equal(gsteps[o], steps)
+    ok = st == 1
+    if precision == "fp64":
+        assert np.array_equal(d["weight"][o][ok].view(np.int64), w[ok].view(np.int64))
+    else:
+        assert np.abs(d["weight"][o][ok] - w[ok]).max() < FP32_TOL
+        _say("fp64 fallbacks", s["fp64_fallbacks"], "of", 6 * s["walk_iterations"], "faces")
+    assert s["located"] == ok.sum() and s["lost"] == (~ok).sum()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_locate_from_previous_cell_and_max_traversal(swarm_cls, oracle, precision):
+    _say("locate_prev_cell", precision)
+    IM, JM, KM = 2, 3, 1100
+    coor = W.uniform_grid(IM, JM, KM)
+    rng = np.random.default_rng(1)
+    pos = rng.random((5000, 3)) * [0.98, 0.98, 1.0] + [0.01, 0.01, 0.0]
+    prev = np.floor(pos * [IM, JM, KM]).astype(np.int64) + rng.integers(-3, 4, (5000, 3))
+    prev[:2500] = -1  # half start from (0,0,0): k >= 1000 is LOST by the cap
+    status = np.zeros(5000, np.int64)
+    cell, w, st, steps = oracle.locate(IM, JM, KM, coor, pos, cell=prev, status=status)
+    assert (st == 3).sum() > 100
+    # prev cells outside the grid fall back to (0,0,0) in both
+    with swarm_cls(precision=precision, collect_walk_stats=True) as g:
+        g.upload_grid(IM, JM, KM, coor)
+        g.upload_particles(pos, cell=prev, status=status)
+        g.LocateAllParticlesInGrid()
+        gsteps = g.walk_steps()
+        raw = g.download_particles()
+    o = np.argsort(raw["pid"])
+    d = {k: v[o] for k, v in raw.items()}
+    np.testing.assert_array_equal(d["status"], st)
+    np.testing.assert_array_equal(d["cell"], cell)
+    np.testing.assert_array_equal(gsteps[o], steps)
+    ok = st == 1
+    if precision == "fp64":
+        assert np.array_equal(d["weight"][ok], w[ok])
+
+
+def test_degenerate_grid_is_an_error(swarm_cls):
+    _say("degenerate")
+    coor = W.uniform_grid(1, 1, 1)
+    coor[:] = 0.0
+    with swarm_cls() as g:
+        g.upload_grid(1, 1, 1, coor)
+        g.upload_particles(np.zeros((1, 3)))
+        with pytest.raises(L.PicgpuError) as e:
+            g.LocateAllParticlesInGrid()
+        assert e.value.code == -3
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("grid", ["warped", "tube"])
+def test_interpolate(swarm_cls, oracle, grid, precision):
+    _say("interpolate", grid, precision)
+    IM, JM, KM, coor = GRIDS[grid]()
+    ucat = (W.ucat_poiseuille(coor, IM, JM, KM, seed=5) if grid == "tube"
+            else W.ucat_taylor_green(coor, IM, JM, KM))
+    pos = _points(grid, 20000, 3)
+    cell, w, st, _ = oracle.locate(IM, JM, KM, coor, pos)
+    corner = oracle.center_to_corner(IM, JM, KM, ucat)
+    vel = oracle.interp(IM, JM, KM, corner, cell, w)
+    with swarm_cls(precision=precision) as g:
+        g.upload_grid(IM, JM, KM, coor)
+        g.upload_ucat(ucat)
+        g.upload_particles(pos, cell=cell, weight=w, status=st)
+        g.InterpolateAllFieldsToSwarm()
+        gc = g.field(L.FIELD_CORNER_VELOCITY)
+        d = _by_pid(g.download_particles())
+    scale = np.abs(ucat).max()
+    if precision == "fp64":
+        assert np.array_equal(gc, corner.ravel())
+        assert np.array_equal(d["vel"], vel)
+    else:
+        assert np.abs(gc - corner.ravel()).max() <= FP32_TOL * scale
+        assert np.abs(d["vel"] - vel).max() <= FP32_TOL * scale
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_advect_and_removal(swarm_cls, oracle, precision):
+    _say("advect_remove", precision)
+    IM, JM, KM, coor = GRIDS["warped"]()
+    rng = np.random.default_rng(4)
+    pos = rng.random((10000, 3))
+    vel = rng.normal(0, 0.1, (10000, 3))
+    cell, w, st, _ = oracle.locate(IM, JM, KM, coor, pos)
+    with swarm_cls(precision=precision) as g:
+        g.upload_grid(IM, JM, KM, coor)
+        g.upload_particles(pos, vel=vel, cell=cell, weight=w, status=st)
+        g.ResetAllParticleStatuses()
+        g.UpdateAllParticlePositions(0.37)
+        d = _by_pid(g.download_particles())
+        vv = vel.astype(np.float32).astype(np.float64) if precision == "fp32" else vel
+        exp = pos.copy()
+        exp += vv * 0.37
+        assert np.array_equal(d["pos"], exp)
+        assert (d["status"] == np.where(st == 1, 0, st)).all()
+        g.LocateAllParticlesInGrid()
+        n0 = g.n
+        rl = g.CheckAndRemoveLostParticles()
+        ro = g.CheckAndRemoveOutOfBoundsParticles()
+        d2 = _by_pid(g.download_particles())
+    c2, w2, st2, _ = oracle.locate(IM, JM, KM, coor, exp, cell=cell, status=np.zeros(10000, np.int64))
+    lo, hi = oracle.bbox(IM, JM, KM, coor)
+    inbox = ((exp >= lo) & (exp <= hi)).all(1)
+    keep = (st2 != 3) & inbox
+    assert rl == (st2 == 3).sum() and ro == ((st2 != 3) & ~inbox).sum()
+    assert n0 - rl - ro == keep.sum() == len(d2["pid"])
+    np.testing.assert_array_equal(d2["pid"], np.arange(10000)[keep])
+    np.testing.assert_array_equal(d2["cell"], c2[keep])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_scatter_and_faces(swarm_cls, oracle, precision):
+    _say("scatter_faces", precision)
+    IM, JM, KM, coor = GRIDS["warped"]()
+    rng = np.random.default_rng(6)
+    pos = rng.random((30000, 3))
+    cell, w, st, _ = oracle.locate(IM, JM, KM, coor, pos)
+    vel = rng.normal(0, 1.0, (30000, 3))
+    P = 101325.0 + rng.normal(0, 10.0, 30000)
+    pid = rng.permutation(30000).astype(np.int64) * 3 + 11
+    cnt = oracle.count_cells(IM, JM, KM, cell)
+    pav = oracle.scatter_P(IM, JM, KM, cell, P, pid, cnt)
+    fi, fj, fk = oracle.deposit_faces(IM, JM, KM, cell, w, vel, pid)
+    with swarm_cls(precision=precision) as g:
+        g.upload_grid(IM, JM, KM, coor)
+        g.upload_particles(pos, vel=vel, cell=cell, weight=w, status=st, pid=pid, P=P)
+        g.ScatterAllParticleFieldsToEulerFields()
+        g.DepositParticlesToFaces()
+        gcnt = g.field(L.FIELD_PARTICLE_COUNT)
+        gpav = g.field(L.FIELD_P_AVERAGE)
+        gf = [g.field(L.FIELD_IFACE), g.field(L.FIELD_JFACE), g.field(L.FIELD_KFACE)]
+        gf2 = [g.field(L.FIELD_IFACE), g.field(L.FIELD_JFACE), g.field(L.FIELD_KFACE)]
+        n = g.n
+    assert n == 30000
+    assert np.array_equal(gcnt, cnt)
+    if precision == "fp64":
+        assert np.array_equal(gpav, pav)
+        for a, b in zip(gf, (fi, fj, fk)):
+            assert np.array_equal(a, b)
+    else:
+        assert np.abs(gpav - pav).max() <= FP32_TOL * 101325.0
+        for a, b in zip(gf, (fi, fj, fk)):
+            a4, b4 = a.reshape(-1, 4), b.reshape(-1, 4)
+            assert np.abs(a4[:, 0] - b4[:, 0]).max() <= FP32_TOL * max(1.0, np.abs(b4[:, 0]).max())
+            assert np.abs(a4[:, 1:] - b4[:, 1:]).max() <= FP32_TOL * max(1.0, np.abs(b4[:, 1:]).max())
+    for a, b in zip(gf, gf2):
+        assert np.array_equal(a, b)
+
+
+def _run_oracle_steps(oracle, d, nsteps, rk2):
+    IM, JM, KM, coor = d["IM"], d["JM"], d["KM"], d["coor"]
+    corner = oracle.center_to_corner(IM, JM, KM, d["ucat"])
+    n = d["pos"].shape[0]
+    s = {"pos": d["pos"].copy(), "vel": np.zeros((n, 3)), "cell": np.full((n, 3), -1, np.int64),
+         "weight": np.zeros((n, 3)), "status": np.full(n, 4, np.int64), "pid": np.arange(n, dtype=np.int64),
+         "P": np.full(n, 101325.0)}
+    # initial setup (simulation.c:129-175): locate, interpolate
+    c, w, st, _ = oracle.locate(IM, JM, KM, coor, s["pos"], status=s["status"])
+    s.update(cell=c, weight=w, status=st)
+    s["vel"] = oracle.interp(IM, JM, KM, corner, c, w)
+    for _ in range(nsteps):
+        oracle.step(IM, JM, KM, coor, corner, s, d["dt"], rk2=rk2)
+    return s, corner
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", ["config1", "tube_rk2", "warped_rk2"])
+def test_full_steps_against_oracle(swarm_cls, oracle, case, precision):
+    _say("full_steps", case, precision)
+    if case == "config1":
+        d = W.config(1)
+        nsteps, rk2 = 10, False
+    elif case == "tube_rk2":
+        d = W.config(2, scale=0.25)
+        d["pos"] = d["pos"][:40000]
+        nsteps, rk2 = 6, True
+    else:
+        d = W.config(5, scale=0.0625)
+        d["pos"] = d["pos"][:40000]
+        nsteps, rk2 = 5, True
+    IM, JM, KM = d["IM"], d["JM"], d["KM"]
+    s, corner = _run_oracle_steps(oracle, d, nsteps, rk2)
+    with swarm_cls(precision=precision) as g:
+        g.upload_grid(IM, JM, KM, d["coor"])
+        g.upload_ucat(d["ucat"])
+        g.upload_particles(d["pos"])
+        g.LocateAllParticlesInGrid()
+        g.InterpolateAllFieldsToSwarm()
+        g.AdvanceSimulation(d["dt"], nsteps, rk2=rk2, deposit_cells=True, deposit_faces=True)
+        out = _by_pid(g.download_particles())
+        cnt = g.field(L.FIELD_PARTICLE_COUNT)
+        faces = [g.field(L.FIELD_IFACE), g.field(L.FIELD_JFACE), g.field(L.FIELD_KFACE)]
+    if precision == "fp64":
+        np.testing.assert_array_equal(out["pid"], s["pid"])
+        np.testing.assert_array_equal(out["cell"], s["cell"])
+        np.testing.assert_array_equal(out["status"], s["status"])
+        assert np.array_equal(out["pos"], s["pos"])
+        assert np.array_equal(out["vel"], s["vel"])
+        assert np.array_equal(out["weight"], s["weight"])
+        assert np.array_equal(cnt, oracle.count_cells(IM, JM, KM, s["cell"]))
+        ref = oracle.deposit_faces(IM, JM, KM, s["cell"], s["weight"], s["vel"], s["pid"])
+        for a, b in zip(faces, ref):
+            assert np.array_equal(a, b)
+    else:
+        # FP32 trajectories drift from FP64 ones; ownership is exact on the
+        # variant's own positions, and the drift is small
+        common, ia, ib = np.intersect1d(out["pid"], s["pid"], return_indices=True)
+        assert len(common) >= 0.99 * len(s["pid"])
+        dpos = np.abs(out["pos"][ia] - s["pos"][ib]).max()
+        _say("fp32 max position drift vs fp64", dpos)
+        assert dpos < 1e-4
+        c2, w2, st2, _ = oracle.locate(IM, JM, KM, d["coor"], out["pos"], cell=out["cell"],
+                                       status=np.zeros(len(out["pid"]), np.int64))
+        # re-locating the FP32 positions from their own cells keeps the cells
+        np.testing.assert_array_equal(c2, out["cell"])
+        assert np.abs(w2 - out["weight"]).max() < FP32_TOL
+        v2 = oracle.interp(IM, JM, KM, corner, c2, w2)
+        assert np.abs(v2 - out["vel"]).max() <= FP32_TOL * np.abs(d["ucat"]).max()
+        assert np.array_equal(cnt, oracle.count_cells(IM, JM, KM, out["cell"]))
