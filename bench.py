@@ -1,0 +1,300 @@
+#!/usr/bin/env python3
+"""Particle-steps/s of PICurv's per-step particle loop on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 5] [--scale 1.0]
+
+A step is one pass of the loop over the whole swarm: reset, advect (RK2 or
+Euler) with the walking-search locate, removal, centre->corner +
+trilinear interpolation, sort by cell and deposition.  Inputs (grid, field,
+particles) are synthesised on the GPU and resident in HBM before timing.
+Two variants run back to back on the same inputs:
+  fp64      plain FP64, the reference arithmetic (bit-exact cells/weights)
+  fp32_mlp  FP32 weights/interpolation/advection/deposition, FP64 positions
+            and near-face decisions, MLP start cell for unlocated particles
+`value` is the fp32_mlp throughput.  One JSON line is printed.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/sec (locate+interp+advect+deposit), FP32+MLP vs plain FP64, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(kernel: str, n: int, g: dict, rsize: int) -> float:
+    """Compulsory HBM bytes of one launch of `kernel` over n particles.
+
+    Particle record terms count each field read or written once; grid terms
+    count each node/cell/face array once.  Sorting scratch and re-reads are
+    excluded (implementation choices)."""
+    Nn, Nc = g["nnode"], g["ncell"]
+    if kernel.startswith("locate"):
+        # read pos(24) cell(4) status(4); write cell(4) status(4) weights(3r); nodes 24 B once
+        return n * (24 + 4 + 4 + 4 + 4 + 3 * rsize) + Nn * 24
+    if kernel == "interp":
+        return n * (4 + 3 * rsize + 3 * rsize) + Nn * 3 * rsize
+    if kernel == "center_to_corner":
+        return Nc * 3 * rsize + Nn * 3 * rsize
+    if kernel == "deposit_faces":
+        # one direction: weights(1r)+velocity(3r)+pid(8) per particle, starts, faces x4
+        nf = g["nface_avg"]
+        return n * (rsize + 3 * rsize + 8) + Nc * 4 + nf * 4 * rsize
+    if kernel == "sort_scatter":
+        rec = 24 + 3 * rsize + 3 * rsize + 4 + 4 + 8 + rsize
+        return n * (2 * rec + 8)
+    if kernel in ("rk2_predict",):
+        return n * (24 + 3 * rsize + 4 + 24 + 3 * rsize + 24)
+    if kernel in ("rk2_correct",):
+        return n * (24 + 3 * rsize + 3 * rsize + 4 + 24 + 4)
+    if kernel == "compact":
+        rec = 24 + 3 * rsize + 3 * rsize + 4 + 4 + 8 + rsize
+        return n * (2 * rec + 8)
+    return float("nan")
+
+
+def step_bytes(n: int, g: dict, rsize: int, faces: bool) -> float:
+    """Per-step algorithmic bytes B = N*2S + Nn*(24+c) + Ncell*c + Nn*c + D."""
+    S = 32 if rsize == 4 else 56
+    c = 3 * rsize
+    D = g["nface"] * 4 * rsize if faces else g["ncell"] * 2 * rsize
+    return n * 2 * S + g["nnode"] * (24 + c) + g["ncell"] * c + g["nnode"] * c + D
+
+
+def make_inputs(cfg: int, scale: float, device: str):
+    import torch
+
+    from fdf_amd import workloads as W
+
+    xp = W.namespace("torch", device)
+    d = W.config(cfg, scale=scale, xp=xp, with_particles=False)
+    n = d["n"]
+    gen = torch.Generator(device=device)
+    gen.manual_seed(1000 + cfg)
+    if d["particles"] == "cube":
+        pos = torch.rand((n, 3), dtype=torch.float64, device=device, generator=gen)
+    else:
+        pos = torch.as_tensor(W.particles_uniform_tube(n, 1000 + cfg), device=device)
+    d["pos_dev"] = pos.contiguous()
+    d["coor"] = d["coor"].contiguous()
+    d["ucat"] = d["ucat"].contiguous()
+    return d
+
+
+def run_variant(name: str, d: dict, steps: int, warmup: int, mlp: dict | None):
+    import torch
+
+    from fdf_amd.swarm import PicGpu
+
+    precision = "fp64" if name == "fp64" else "fp32"
+    use_mlp = name.endswith("mlp") and mlp is not None
+    IM, JM, KM = d["IM"], d["JM"], d["KM"]
+    g = PicGpu(precision=precision, use_mlp_guess=False)
+    g.upload_grid_device(IM, JM, KM, d["coor"].data_ptr())
+    g.upload_ucat_device(d["ucat"].data_ptr())
+    n0 = int(d["pos_dev"].shape[0])
+    g.init_particles_device(n0, d["pos_dev"].data_ptr())
+    torch.cuda.synchronize()
+    res = {"variant": name, "precision": precision, "mlp": bool(use_mlp)}
+    if use_mlp:
+        res["mlp_train"] = mlp["train"](g)
+    t0 = time.perf_counter()
+    g.LocateAllParticlesInGrid()
+    st0 = g.stats()
+    g.InterpolateAllFieldsToSwarm()
+    torch.cuda.synchronize()
+    res["initial_locate_s"] = time.perf_counter() - t0
+    res["initial_located"], res["initial_lost"] = st0["located"], st0["lost"]
+    res["initial_walk_iterations_per_particle"] = st0["walk_iterations"] / max(1, n0)
+    log(f"{name}: initial locate {res['initial_locate_s']:.2f}s located={st0['located']} lost={st0['lost']}")
+    flags = dict(rk2=d["rk2"], deposit_cells=("cells" in d["deposit"]), deposit_faces=("faces" in d["deposit"]))
+    for _ in range(warmup):
+        g.AdvanceSimulation(d["dt"], 1, **flags)
+    torch.cuda.synchronize()
+    g.timing(True)
+    g.timing_reset()
+    ps0 = g.stats()["particle_steps"]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.AdvanceSimulation(d["dt"], steps, **flags)
+    g.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    st = g.stats()
+    res["seconds"] = t1 - t0
+    res["particle_steps"] = st["particle_steps"] - ps0
+    res["value"] = res["particle_steps"] / res["seconds"]
+    res["ms_per_step"] = 1e3 * res["seconds"] / max(1, steps)
+    res["n_end"] = st["n_particles"]
+    res["walk_iterations_last_locate_per_particle"] = st["walk_iterations"] / max(1, st["n_particles"])
+    res["fp64_fallback_faces_last_locate"] = st["fp64_fallbacks"]
+    kernels = {}
+    for k in ("locate_fp64", "locate_fp32", "interp", "center_to_corner", "rk2_predict", "rk2_correct",
+              "keep_flags", "compact", "cell_histogram", "sort_scatter", "scan_tile", "scan_add", "scatter_P",
+              "deposit_faces", "reset_advect", "compact_tail", "keep_unlocated"):
+        ms, cnt = g.timing_query(k)
+        if cnt:
+            kernels[k] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt}
+    res["kernels"] = kernels
+    g.timing(False)
+    g.close()
+    log(f"{name}: {res['value']:.4g} particle-steps/s, {res['ms_per_step']:.3f} ms/step")
+    return res
+
+
+def roofline_for(res: dict, d: dict, g: dict) -> dict:
+    rsize = 8 if res["precision"] == "fp64" else 4
+    kern = res["kernels"]
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    n_avg = res["particle_steps"] / max(1, res["steps"])
+    b = algorithmic_bytes(dom, int(n_avg), g, rsize)
+    achieved = b / (kern[dom]["avg_ms"] * 1e-3) / 1e9
+    return {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "algorithmic_bytes_per_launch": b, "avg_launch_ms": kern[dom]["avg_ms"],
+            "share_of_step_time": kern[dom]["total_ms"] / (1e3 * res["seconds"])}
+
+
+def cpu_baseline(d: dict, sample_particles: int, sample_steps: int) -> dict:
+    """The FP64 oracle built -O3 with OpenMP, on a bounded sample of the same
+    workload (full grid and field, a subset of the particles)."""
+    import numpy as np
+
+    from oracle.oracle import Oracle
+
+    threads = min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    o = Oracle(fast=True)
+    IM, JM, KM = d["IM"], d["JM"], d["KM"]
+    coor = d["coor"].cpu().numpy().reshape(-1, 3)
+    ucat = d["ucat"].cpu().numpy().reshape(-1, 3)
+    pos = d["pos_dev"][:sample_particles].cpu().numpy()
+    n = pos.shape[0]
+    corner = o.center_to_corner(IM, JM, KM, ucat)
+    # start from located particles (the initial setup is not timed)
+    cell, w, st, _ = o.locate(IM, JM, KM, coor, pos)
+    s = {"pos": pos.copy(), "vel": o.interp(IM, JM, KM, corner, cell, w), "cell": cell, "weight": w,
+         "status": st, "pid": np.arange(n, dtype=np.int64), "P": np.full(n, 101325.0)}
+    faces = "faces" in d["deposit"]
+    t0 = time.perf_counter()
+    ps = 0
+    for _ in range(sample_steps):
+        ps += s["pos"].shape[0]
+        corner = o.center_to_corner(IM, JM, KM, ucat)
+        o.step(IM, JM, KM, coor, corner, s, d["dt"], rk2=d["rk2"])
+        if faces:
+            o.deposit_faces(IM, JM, KM, s["cell"], s["weight"], s["vel"], s["pid"])
+        else:
+            o.count_cells(IM, JM, KM, s["cell"])
+    dt = time.perf_counter() - t0
+    return {"value": ps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"config {d['config']} full grid {IM}x{JM}x{KM}, {n} of {d['n']} particles, "
+                      f"{sample_steps} steps ({'RK2' if d['rk2'] else 'Euler'}, "
+                      f"{'face' if faces else 'cell'} deposition, centre->corner each step), "
+                      f"FP64 oracle -O3 x86-64-v3 OpenMP, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=5)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--variants", default="fp32_mlp,fp64")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 19)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from fdf_amd import _lib
+
+    _lib.load()
+    dev = "cuda:0"
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    d = make_inputs(args.config, args.scale, dev)
+    torch.cuda.synchronize()
+    log(f"inputs: config {args.config} {d['IM']}x{d['JM']}x{d['KM']} cells, {d['n']} particles, "
+        f"{time.perf_counter() - t0:.1f}s")
+    steps = args.steps if args.steps is not None else min(d["steps"], 100)
+    g = {"nnode": (d["IM"] + 1) * (d["JM"] + 1) * (d["KM"] + 1), "ncell": d["IM"] * d["JM"] * d["KM"]}
+    g["nface"] = ((d["IM"] + 1) * d["JM"] * d["KM"] + d["IM"] * (d["JM"] + 1) * d["KM"] +
+                  d["IM"] * d["JM"] * (d["KM"] + 1))
+    g["nface_avg"] = g["nface"] / 3
+
+    mlp = None
+    try:
+        from fdf_amd import mlp as M
+
+        mlp = {"train": lambda ctx: M.train_on_swarm(ctx, d)}
+    except Exception as e:  # the MLP module is optional in this bench line only
+        log("mlp unavailable:", e)
+
+    results = {}
+    for v in [s.strip() for s in args.variants.split(",") if s.strip()]:
+        r = run_variant(v, d, steps, args.warmup, mlp)
+        r["steps"] = steps
+        r["roofline"] = roofline_for(r, d, g)
+        rs = 8 if r["precision"] == "fp64" else 4
+        n_avg = r["particle_steps"] / max(1, steps)
+        r["step_roofline"] = {"algorithmic_bytes_per_step": step_bytes(int(n_avg), g, rs, "faces" in d["deposit"]),
+                              "achieved_GBs": step_bytes(int(n_avg), g, rs, "faces" in d["deposit"]) /
+                              (r["ms_per_step"] * 1e-3) / 1e9}
+        r["step_roofline"]["frac"] = r["step_roofline"]["achieved_GBs"] / HBM_PEAK_GBS
+        results[v] = r
+        torch.cuda.empty_cache()
+
+    head = results.get("fp32_mlp") or next(iter(results.values()))
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(d, min(args.cpu_sample, d["n"]), args.cpu_steps)
+            log(f"cpu baseline: {cpu['value']:.4g} particle-steps/s on {cpu['cores']} cores")
+        except Exception as e:
+            log("cpu baseline failed:", e)
+    line = {
+        "metric": METRIC,
+        "value": head["value"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head["precision"] == "fp32" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"config{args.config}", "grid_cells": [d["IM"], d["JM"], d["KM"]],
+                   "particles": d["n"], "scale": args.scale, "advection": "RK2" if d["rk2"] else "Euler",
+                   "deposition": d["deposit"], "dt": d["dt"], "variant": head["variant"]},
+        "roofline": {k: head["roofline"][k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
+        "roofline_kernel": head["roofline"]["kernel"],
+        "cpu_baseline": cpu,
+        "variants": {k: {kk: vv for kk, vv in v.items() if kk not in ("kernels",)} for k, v in results.items()},
+        "kernels": {k: v["kernels"] for k, v in results.items()},
+    }
+    if "fp64" in results and "fp32_mlp" in results:
+        line["fp32_mlp_over_fp64"] = results["fp32_mlp"]["value"] / results["fp64"]["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

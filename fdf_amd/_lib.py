@@ -79,6 +79,9 @@ _SIG = {
     "picgpu_upload_grid": (C.c_int, [_P, _I64, _I64, _I64, _P]),
     "picgpu_upload_ucat": (C.c_int, [_P, _P]),
     "picgpu_upload_particles": (C.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _P]),
+    "picgpu_upload_grid_device": (C.c_int, [_P, _I64, _I64, _I64, _P]),
+    "picgpu_upload_ucat_device": (C.c_int, [_P, _P]),
+    "picgpu_init_particles_device": (C.c_int, [_P, _I64, _P, _I64]),
     "picgpu_num_particles": (C.c_int, [_P, C.POINTER(_I64)]),
     "picgpu_download_particles": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "picgpu_download_walk_steps": (C.c_int, [_P, _P]),

@@ -528,6 +528,81 @@ __global__ __launch_bounds__(kBlock) void k_face_volfrac(int dir, GridView g, co
   out[f] = ((double)face[4 * f] * vp) / vd;
 }
 
+// ---------------------------------------------------------------------------
+// Device-side input preparation.
+// ---------------------------------------------------------------------------
+// per-block min/max of node coordinates: out[6*b .. 6*b+5] = (min xyz, max xyz)
+__global__ __launch_bounds__(kBlock) void k_bbox_partial(int64_t nnode, const double *__restrict__ coor,
+                                                         double *__restrict__ out) {
+  __shared__ double sh[6][kBlock];
+  double lo[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+  double hi[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nnode; q += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      double x = coor[3 * q + a];
+      lo[a] = x < lo[a] ? x : lo[a];
+      hi[a] = x > hi[a] ? x : hi[a];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    sh[a][threadIdx.x] = lo[a];
+    sh[3 + a][threadIdx.x] = hi[a];
+  }
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        double o = sh[a][threadIdx.x + s];
+        sh[a][threadIdx.x] = o < sh[a][threadIdx.x] ? o : sh[a][threadIdx.x];
+        double h = sh[3 + a][threadIdx.x + s];
+        sh[3 + a][threadIdx.x] = h > sh[3 + a][threadIdx.x] ? h : sh[3 + a][threadIdx.x];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[6 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// node-slot ucat (double) -> compact [KM][JM][IM][3] in the working precision
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_ucat_compact(int IM, int JM, int KM, const double *__restrict__ in,
+                                                         R *__restrict__ out) {
+  const int64_t ncell = (int64_t)IM * JM * KM;
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int i = (int)(c % IM);
+  int64_t r = c / IM;
+  int j = (int)(r % JM);
+  int k = (int)(r / JM);
+  const double *u = in + 3 * (((int64_t)k * (JM + 1) + j) * (IM + 1) + i);
+  out[3 * c] = (R)u[0];
+  out[3 * c + 1] = (R)u[1];
+  out[3 * c + 2] = (R)u[2];
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_init_particles(int64_t n, const double *__restrict__ pos_in,
+                                                           int64_t pid_base, double *__restrict__ pos,
+                                                           R *__restrict__ vel, R *__restrict__ w,
+                                                           int32_t *__restrict__ cell, int32_t *__restrict__ status,
+                                                           int64_t *__restrict__ pid, R *__restrict__ P) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    pos[3 * p + c] = pos_in[3 * p + c];
+    vel[3 * p + c] = (R)0;
+    w[3 * p + c] = (R)0;
+  }
+  cell[p] = -1;
+  status[p] = ST_UNINIT;
+  pid[p] = pid_base + p;
+  P[p] = (R)101325.0;
+}
+
 __global__ __launch_bounds__(kBlock) void k_keep_unlocated(int64_t n, const int32_t *__restrict__ cell,
                                                            int32_t *__restrict__ keep) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

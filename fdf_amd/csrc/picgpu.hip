@@ -111,6 +111,41 @@ static int need_grid(picgpu_ctx *c) {
 
 static GridView grid_view(picgpu_ctx *c) { return GridView{c->coor.as<double>(), c->IM, c->JM, c->KM}; }
 
+// Padded bounding box of all nodes (grid.c:762-883 on one rank) and state reset
+// after a grid upload.
+static int finish_grid(picgpu_ctx *c) {
+  const int64_t nb = std::min<int64_t>(1024, nblocks(c->nnode));
+  DevBuf part;
+  PG_CHECK(part.ensure(sizeof(double) * 6 * (size_t)nb));
+  PG_LAUNCH(c, "bbox", nb, kBlock, k_bbox_partial, c->nnode, c->coor.as<double>(), part.as<double>());
+  std::vector<double> h(6 * (size_t)nb);
+  hipError_t e1 = hipMemcpyAsync(h.data(), part.ptr, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream);
+  hipError_t e2 = hipStreamSynchronize(c->stream);
+  part.release();
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    set_error("bounding box reduction failed");
+    return PICGPU_E_HIP;
+  }
+  for (int a = 0; a < 3; ++a) {
+    c->bmin[a] = 1.7976931348623157e308;
+    c->bmax[a] = -1.7976931348623157e308;
+  }
+  for (int64_t b = 0; b < nb; ++b)
+    for (int a = 0; a < 3; ++a) {
+      c->bmin[a] = std::min(c->bmin[a], h[6 * b + a]);
+      c->bmax[a] = std::max(c->bmax[a], h[6 * b + 3 + a]);
+    }
+  for (int a = 0; a < 3; ++a) {
+    c->bmin[a] = c->bmin[a] - c->opt.bbox_tolerance;
+    c->bmax[a] = c->bmax[a] + c->opt.bbox_tolerance;
+  }
+  c->have_grid = true;
+  c->have_ucat = false;
+  c->sorted = false;
+  c->have_pavg = c->have_faces = false;
+  return PICGPU_OK;
+}
+
 // ---------------------------------------------------------------------------
 // lifetime
 // ---------------------------------------------------------------------------
@@ -230,25 +265,69 @@ int picgpu_upload_grid(picgpu_ctx *c, int64_t IM, int64_t JM, int64_t KM, const 
   c->nnode = (IM + 1) * (JM + 1) * (KM + 1);
   PG_CHECK(c->coor.ensure(sizeof(double) * 3 * (size_t)c->nnode));
   PG_HIP(hipMemcpy(c->coor.ptr, coor, sizeof(double) * 3 * (size_t)c->nnode, hipMemcpyHostToDevice));
-  // padded bounding box (grid.c:762-883 on one rank)
-  for (int a = 0; a < 3; ++a) {
-    c->bmin[a] = 1.7976931348623157e308;
-    c->bmax[a] = -1.7976931348623157e308;
+  return finish_grid(c);
+}
+
+int picgpu_upload_grid_device(picgpu_ctx *c, int64_t IM, int64_t JM, int64_t KM, const double *d_coor) {
+  if (!c || !d_coor || IM <= 0 || JM <= 0 || KM <= 0) {
+    set_error("upload_grid_device: bad arguments");
+    return PICGPU_E_ARG;
   }
-  for (int64_t q = 0; q < c->nnode; ++q)
-    for (int a = 0; a < 3; ++a) {
-      double x = coor[3 * q + a];
-      if (x < c->bmin[a]) c->bmin[a] = x;
-      if (x > c->bmax[a]) c->bmax[a] = x;
-    }
-  for (int a = 0; a < 3; ++a) {
-    c->bmin[a] = c->bmin[a] - c->opt.bbox_tolerance;
-    c->bmax[a] = c->bmax[a] + c->opt.bbox_tolerance;
+  if (IM * JM * KM >= (int64_t)INT32_MAX - 2) {
+    set_error("upload_grid_device: grid too large for 32-bit packed cell indices");
+    return PICGPU_E_UNSUPPORTED;
   }
-  c->have_grid = true;
-  c->have_ucat = false;
+  PG_HIP(hipSetDevice(c->device));
+  c->IM = (int)IM;
+  c->JM = (int)JM;
+  c->KM = (int)KM;
+  c->ncell = IM * JM * KM;
+  c->nnode = (IM + 1) * (JM + 1) * (KM + 1);
+  PG_CHECK(c->coor.ensure(sizeof(double) * 3 * (size_t)c->nnode));
+  PG_HIP(hipMemcpyAsync(c->coor.ptr, d_coor, sizeof(double) * 3 * (size_t)c->nnode, hipMemcpyDeviceToDevice,
+                        c->stream));
+  return finish_grid(c);
+}
+
+int picgpu_upload_ucat_device(picgpu_ctx *c, const double *d_ucat) {
+  if (!c || !d_ucat) return PICGPU_E_ARG;
+  PG_CHECK(need_grid(c));
+  PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(c->ucat.ensure(c->rsize * 3 * (size_t)c->ncell));
+  PG_CHECK(c->corner.ensure(c->rsize * 3 * (size_t)c->nnode));
+  int r = dispatch_real(c, [&](auto *tag) -> int {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    PG_LAUNCH(c, "ucat_compact", nblocks(c->ncell), kBlock, k_ucat_compact<R>, c->IM, c->JM, c->KM, d_ucat,
+              c->ucat.as<R>());
+    return PICGPU_OK;
+  });
+  PG_CHECK(r);
+  PG_HIP(hipStreamSynchronize(c->stream));
+  c->have_ucat = true;
+  return PICGPU_OK;
+}
+
+int picgpu_init_particles_device(picgpu_ctx *c, int64_t n, const double *d_pos, int64_t pid_base) {
+  if (!c || n < 0 || (n > 0 && !d_pos)) return PICGPU_E_ARG;
+  PG_CHECK(need_grid(c));
+  if (n >= (int64_t)INT32_MAX) {
+    set_error("init_particles_device: more than 2^31-1 particles");
+    return PICGPU_E_UNSUPPORTED;
+  }
+  PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(ensure_swarm(c, c->sw, n));
+  c->n = n;
   c->sorted = false;
-  c->have_pavg = c->have_faces = false;
+  if (n == 0) return PICGPU_OK;
+  int r = dispatch_real(c, [&](auto *tag) -> int {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    PG_LAUNCH(c, "init_particles", nblocks(n), kBlock, k_init_particles<R>, n, d_pos, pid_base,
+              c->sw.pos.as<double>(), c->sw.vel.as<R>(), c->sw.w.as<R>(), c->sw.cell.as<int32_t>(),
+              c->sw.status.as<int32_t>(), c->sw.pid.as<int64_t>(), c->sw.P.as<R>());
+    return PICGPU_OK;
+  });
+  PG_CHECK(r);
+  PG_HIP(hipStreamSynchronize(c->stream));
   return PICGPU_OK;
 }
 

@@ -86,6 +86,17 @@ class PicGpu:
         L.check(self.lib.picgpu_upload_particles(self.h, n, _ptr(pos), _ptr(vel), _ptr(cell), _ptr(weight),
                                                   _ptr(status), _ptr(pid), _ptr(P)))
 
+    # device-resident inputs (HIP device pointers, e.g. torch.Tensor.data_ptr())
+    def upload_grid_device(self, IM: int, JM: int, KM: int, d_coor: int):
+        L.check(self.lib.picgpu_upload_grid_device(self.h, IM, JM, KM, C.c_void_p(d_coor)))
+        self.IM, self.JM, self.KM = IM, JM, KM
+
+    def upload_ucat_device(self, d_ucat: int):
+        L.check(self.lib.picgpu_upload_ucat_device(self.h, C.c_void_p(d_ucat)))
+
+    def init_particles_device(self, n: int, d_pos: int, pid_base: int = 0):
+        L.check(self.lib.picgpu_init_particles_device(self.h, n, C.c_void_p(d_pos), pid_base))
+
     # ---- outputs ----
     @property
     def n(self) -> int:

@@ -266,10 +266,11 @@ def ucat_random_fourier(coor, IM, JM, KM, seed, nmodes=64, kmax=8, xp=None):
     cc = cell_centroids(coor, IM, JM, KM)
     u = xp.zeros((KM, JM, IM, 3))
     for m in range(nmodes):
-        ph = 2 * math.pi * (ks[m, 0] * cc[..., 0] + ks[m, 1] * cc[..., 1] + ks[m, 2] * cc[..., 2]) + phis[m]
+        k0, k1, k2 = (float(v) for v in ks[m])
+        ph = 2 * math.pi * (k0 * cc[..., 0] + k1 * cc[..., 1] + k2 * cc[..., 2]) + float(phis[m])
         cph = xp.cos(ph)
         for a in range(3):
-            u[..., a] += amps[m, a] * cph
+            u[..., a] += float(amps[m, a]) * cph
     return to_node_slots(u, IM, JM, KM, xp)
 
 

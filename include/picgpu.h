@@ -111,6 +111,16 @@ int picgpu_upload_grid(picgpu_ctx *ctx, int64_t IM, int64_t JM, int64_t KM, cons
  * is stored at slot (i,j,k) (README.md:242-260); other slots are ignored. */
 int picgpu_upload_ucat(picgpu_ctx *ctx, const double *ucat);
 
+/* Same inputs already resident in device memory (same layouts; pointers are
+ * HIP device pointers on the context's device).  For callers that build the
+ * grid and field on the GPU. */
+int picgpu_upload_grid_device(picgpu_ctx *ctx, int64_t IM, int64_t JM, int64_t KM, const double *d_coor);
+int picgpu_upload_ucat_device(picgpu_ctx *ctx, const double *d_ucat);
+/* Fresh particles at device positions d_pos [n][3]: cell -1, status
+ * UNINITIALIZED, velocity and weights 0, P 101325, PID pid_base + p
+ * (InitializeParticleBasicProperties, ParticleSwarm.c:448-454). */
+int picgpu_init_particles_device(picgpu_ctx *ctx, int64_t n, const double *d_pos, int64_t pid_base);
+
 /* ---- swarm I/O (DMSwarm field layout, ParticleSwarm.c:64-96) ---- */
 /* pos, vel, weight: [n][3]; cell: [n][3] int64 (-1 = none); status: [n] int64;
  * pid: [n] int64; P: [n] (NULL -> 101325.0, ParticleSwarm.c:514-518).

@@ -239,7 +239,7 @@ def test_full_steps_against_oracle(swarm_cls, oracle, case, precision):
     _say("full_steps", case, precision)
     if case == "config1":
         d = W.config(1)
-        nsteps, rk2 = 10, False
+        nsteps, rk2 = (10 if precision == "fp64" else 5), False
     elif case == "tube_rk2":
         d = W.config(2, scale=0.25)
         d["pos"] = d["pos"][:40000]
@@ -260,6 +260,9 @@ def test_full_steps_against_oracle(swarm_cls, oracle, case, precision):
         out = _by_pid(g.download_particles())
         cnt = g.field(L.FIELD_PARTICLE_COUNT)
         faces = [g.field(L.FIELD_IFACE), g.field(L.FIELD_JFACE), g.field(L.FIELD_KFACE)]
+    if case == "config1" and nsteps == 10:
+        # z moves by 10 x 0.1: every particle leaves the unit cube (simulation.c:263-265 removes them)
+        assert len(s["pid"]) <= 1
     if precision == "fp64":
         np.testing.assert_array_equal(out["pid"], s["pid"])
         np.testing.assert_array_equal(out["cell"], s["cell"])
@@ -275,7 +278,7 @@ def test_full_steps_against_oracle(swarm_cls, oracle, case, precision):
         # FP32 trajectories drift from FP64 ones; ownership is exact on the
         # variant's own positions, and the drift is small
         common, ia, ib = np.intersect1d(out["pid"], s["pid"], return_indices=True)
-        assert len(common) >= 0.99 * len(s["pid"])
+        assert len(s["pid"]) > 1000 and len(common) >= 0.99 * len(s["pid"])
         dpos = np.abs(out["pos"][ia] - s["pos"][ib]).max()
         _say("fp32 max position drift vs fp64", dpos)
         assert dpos < 1e-4
