@@ -39,6 +39,15 @@ def algorithmic_bytes(kernel: str, n: int, g: dict, rsize: int) -> float:
     count each node/cell/face array once.  Sorting scratch and re-reads are
     excluded (implementation choices)."""
     Nn, Nc = g["nnode"], g["ncell"]
+    if kernel.startswith("step_fused"):
+        # read pos(24) vel(3r) cell(4) status(4); write pos(24) vel(3r) weights(3r) cell(4) status(4) slot(4);
+        # node coordinates (24 B) and corner velocity (3r) once
+        return n * (24 + 3 * rsize + 4 + 4 + 24 + 3 * rsize + 3 * rsize + 4 + 4 + 4) + Nn * (24 + 3 * rsize)
+    if kernel == "index_scatter":
+        return n * (4 + 4 + 4 + 4)
+    if kernel == "gather_swarm":
+        rec = 24 + 3 * rsize + 3 * rsize + 4 + 4 + 8 + rsize
+        return n * (2 * rec + 4)
     if kernel.startswith("locate"):
         # read pos(24) cell(4) status(4); write cell(4) status(4) weights(3r); nodes 24 B once
         return n * (24 + 4 + 4 + 4 + 4 + 3 * rsize) + Nn * 24
@@ -139,9 +148,10 @@ def run_variant(name: str, d: dict, steps: int, warmup: int, mlp: dict | None):
     res["walk_iterations_last_locate_per_particle"] = st["walk_iterations"] / max(1, st["n_particles"])
     res["fp64_fallback_faces_last_locate"] = st["fp64_fallbacks"]
     kernels = {}
-    for k in ("locate_fp64", "locate_fp32", "interp", "center_to_corner", "rk2_predict", "rk2_correct",
-              "keep_flags", "compact", "cell_histogram", "sort_scatter", "scan_tile", "scan_add", "scatter_P",
-              "deposit_faces", "reset_advect", "compact_tail", "keep_unlocated"):
+    for k in ("step_fused_fp64", "step_fused_fp32", "index_scatter", "gather_swarm", "locate_fp64", "locate_fp32",
+              "interp", "center_to_corner", "rk2_predict", "rk2_correct", "keep_flags", "compact", "cell_histogram",
+              "sort_scatter", "scan_tile", "scan_add", "scatter_P", "deposit_faces", "reset_advect", "compact_tail",
+              "keep_unlocated"):
         ms, cnt = g.timing_query(k)
         if cnt:
             kernels[k] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt}

@@ -33,6 +33,7 @@ FIELD_IFACE_VOLFRAC, FIELD_JFACE_VOLFRAC, FIELD_KFACE_VOLFRAC = 6, 7, 8
 STEP_RK2 = 0x1
 STEP_DEPOSIT_CELLS = 0x2
 STEP_DEPOSIT_FACES = 0x4
+STEP_STAGED = 0x100
 
 
 class PicgpuOptions(C.Structure):

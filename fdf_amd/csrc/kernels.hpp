@@ -21,6 +21,27 @@ __device__ __forceinline__ int32_t pack_cell(int i, int j, int k, int IM, int JM
   return (int32_t)(i + IM * (j + JM * k));
 }
 
+// Cycle detection for the walk (Brent).  For a fixed point the walk is a
+// deterministic map cell -> next cell (walkingsearch.c:882-959), so a
+// revisited cell means the walk is periodic: it will never accept a cell
+// and the reference ends it LOST at MAX_TRAVERSAL (:1206-1212).  Warped
+// k-faces leave thin gaps in which walks cycle among two or three cells;
+// concluding them at the first revisit gives the reference's outcome
+// without holding a 64-lane wavefront for up to 1000 iterations.
+struct CycleGuard {
+  int32_t saved = -1;
+  int power = 1, lam = 0;
+  __device__ __forceinline__ bool revisit(int32_t c) {
+    if (c == saved) return true;
+    if (++lam == power) {
+      saved = c;
+      power <<= 1;
+      lam = 0;
+    }
+    return false;
+  }
+};
+
 // ---------------------------------------------------------------------------
 // ResetAllParticleStatuses (ParticleMotion.c:2149-2176) fused with
 // UpdateAllParticlePositions (ParticleMotion.c:19-95): x += v*dt.
@@ -73,6 +94,7 @@ __global__ __launch_bounds__(kBlock) void k_locate(LocateArgs<R> a) {
   int64_t cursor = base;
   int64_t p = -1;
   int ci = 0, cj = 0, ck = 0, steps = 0;
+  CycleGuard cyc;
   double x[3] = {0.0, 0.0, 0.0};
   unsigned long long my_iters = 0, my_max = 0, my_fb = 0, my_loc = 0, my_lost = 0;
   int my_err = 0;
@@ -100,6 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_locate(LocateArgs<R> a) {
               ck = 0;
             }
             steps = 0;
+            cyc = CycleGuard();
           } else if (a.steps_out) {
             a.steps_out[q] = 0;
           }
@@ -117,19 +140,22 @@ __global__ __launch_bounds__(kBlock) void k_locate(LocateArgs<R> a) {
     bool done = false, located = false;
     if (ci < 0 || ci >= a.g.IM || cj < 0 || cj >= a.g.JM || ck < 0 || ck >= a.g.KM) {
       done = true;  // walked outside the global domain: LOST (:1127-1132)
+    } else if (cyc.revisit(pack_cell(ci, cj, ck, a.g.IM, a.g.JM))) {
+      done = true;  // periodic walk: LOST at MAX_TRAVERSAL (:1206-1212)
+      steps = a.max_traversal;
     } else {
-      double V[8][3];
-      load_vertices(a.g, ci, cj, ck, V);
       R d[6];
       bool degenerate;
       if constexpr (EXACT) {
+        double V[8][3];
+        load_vertices(a.g, ci, cj, ck, V);
         double dd[6];
         degenerate = !eval_cell_exact(V, x, a.thresh_scale, dd);
 #pragma unroll
         for (int f = 0; f < 6; ++f) d[f] = dd[f];
       } else {
         float dd[6];
-        FilterResult fr = eval_cell_fp32(V, x, a.thresh_scale, dd);
+        FilterResult fr = eval_cell_fp32(a.g, ci, cj, ck, x, a.thresh_scale, dd);
         degenerate = fr.degenerate;
         my_fb += fr.fallbacks;
 #pragma unroll

@@ -7,7 +7,7 @@
 #include <algorithm>
 #include <vector>
 
-#include "kernels.hpp"
+#include "fused.hpp"
 
 using namespace picgpu;
 
@@ -718,6 +718,128 @@ static int do_rk2(picgpu_ctx *c, double dt) {
 }
 
 // ---------------------------------------------------------------------------
+// fused step: one pass per particle, then an index counting sort by cell and
+// deposition through the index; the swarm is compacted into cell order
+// every kCompactEvery steps and at the end of picgpu_step.
+// ---------------------------------------------------------------------------
+static constexpr int kCompactEvery = 8;
+
+static int compact_by_index(picgpu_ctx *c, int64_t live) {
+  const int64_t n = c->n;
+  PG_CHECK(ensure_swarm(c, c->alt, n));
+  if (live > 0) {
+    int r = dispatch_real(c, [&](auto *tag) -> int {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      PG_LAUNCH(c, "gather_swarm", nblocks(live), kBlock, k_gather_swarm<R>, live, c->rank.as<int32_t>() + n,
+                c->sw.pos.as<double>(), c->sw.vel.as<R>(), c->sw.w.as<R>(), c->sw.cell.as<int32_t>(),
+                c->sw.status.as<int32_t>(), c->sw.pid.as<int64_t>(), c->sw.P.as<R>(), c->alt.pos.as<double>(),
+                c->alt.vel.as<R>(), c->alt.w.as<R>(), c->alt.cell.as<int32_t>(), c->alt.status.as<int32_t>(),
+                c->alt.pid.as<int64_t>(), c->alt.P.as<R>());
+      return PICGPU_OK;
+    });
+    PG_CHECK(r);
+  }
+  swap_swarm(c);
+  c->n = live;
+  return PICGPU_OK;
+}
+
+static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool last) {
+  const int64_t n = c->n;
+  const bool rk2 = (flags & PICGPU_STEP_RK2) != 0;
+  PG_CHECK(counters_reset(c));
+  PG_CHECK(c->cell_count.ensure(sizeof(int32_t) * (size_t)(c->ncell + 2)));
+  PG_CHECK(c->cell_start.ensure(sizeof(int32_t) * (size_t)(c->ncell + 2)));
+  // rank buffer: [0, n) slots, [n, 2n) cell-ordered index
+  PG_CHECK(c->rank.ensure(sizeof(int32_t) * (size_t)(2 * (n > 0 ? n : 1))));
+  PG_HIP(hipMemsetAsync(c->cell_count.ptr, 0, sizeof(int32_t) * (size_t)(c->ncell + 1), c->stream));
+  int32_t *slot = c->rank.as<int32_t>();
+  int32_t *idx = slot + n;
+  if (n > 0) {
+    int r = dispatch_real(c, [&](auto *tag) -> int {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      StepArgs<R> a;
+      a.g = grid_view(c);
+      a.n = n;
+      a.pos = c->sw.pos.as<double>();
+      a.vel = c->sw.vel.as<R>();
+      a.w = c->sw.w.as<R>();
+      a.cell = c->sw.cell.as<int32_t>();
+      a.status = c->sw.status.as<int32_t>();
+      a.corner = c->corner.as<R>();
+      a.dt = dt;
+      a.max_traversal = c->opt.max_traversal;
+      a.thresh_scale = c->opt.distance_threshold;
+      a.clamp = (R)c->opt.weight_clamp;
+      a.bx0 = c->bmin[0];
+      a.by0 = c->bmin[1];
+      a.bz0 = c->bmin[2];
+      a.bx1 = c->bmax[0];
+      a.by1 = c->bmax[1];
+      a.bz1 = c->bmax[2];
+      a.count = c->cell_count.as<int32_t>();
+      a.slot = slot;
+      a.counters = c->counters.as<unsigned long long>();
+      constexpr bool EX = sizeof(R) == sizeof(double);
+      const char *name = EX ? "step_fused_fp64" : "step_fused_fp32";
+      if (rk2) {
+        PG_LAUNCH(c, name, nblocks(n), kBlock, (k_step_fused<R, EX, true>), a);
+      } else {
+        PG_LAUNCH(c, name, nblocks(n), kBlock, (k_step_fused<R, EX, false>), a);
+      }
+      return PICGPU_OK;
+    });
+    PG_CHECK(r);
+  }
+  PG_CHECK(scan_exclusive_i32(c, c->cell_count.as<int32_t>(), c->cell_start.as<int32_t>(), c->ncell));
+  if (n > 0) PG_LAUNCH(c, "index_scatter", nblocks(n), kBlock, k_index_scatter, n, c->sw.cell.as<int32_t>(), slot,
+                       c->cell_start.as<int32_t>(), idx);
+  unsigned long long h[C_NUM];
+  PG_CHECK(counters_read(c, h));
+  if (h[C_ERROR]) {
+    set_error("degenerate cell face (|n| < 1e-12) met during the walk");
+    return PICGPU_E_DEGENERATE;
+  }
+  const int64_t live = (int64_t)h[C_LOCATED];
+  c->stats.located = live;
+  c->stats.removed_lost = (int64_t)h[C_REMOVED_LOST];
+  c->stats.removed_out_of_bounds = (int64_t)h[C_REMOVED_OOB];
+  c->stats.lost = (int64_t)h[C_REMOVED_LOST];
+  c->stats.walk_iterations = (int64_t)h[C_WALK_ITERS];
+  c->stats.fp64_fallbacks = (int64_t)h[C_FALLBACKS];
+  int r = dispatch_real(c, [&](auto *tag) -> int {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    if (flags & PICGPU_STEP_DEPOSIT_CELLS) {
+      PG_CHECK(c->pavg.ensure(sizeof(R) * (size_t)c->ncell));
+      PG_LAUNCH(c, "scatter_P", nblocks(c->ncell), kBlock, k_scatter_P_idx<R>, c->ncell, c->cell_start.as<int32_t>(),
+                idx, c->sw.pid.as<int64_t>(), c->sw.P.as<R>(), c->pavg.as<R>());
+      c->have_pavg = true;
+    }
+    if (flags & PICGPU_STEP_DEPOSIT_FACES) {
+      for (int dir = 0; dir < 3; ++dir) {
+        PG_CHECK(c->face[dir].ensure(sizeof(R) * 4 * (size_t)nface(c, dir)));
+        PG_LAUNCH(c, "deposit_faces", nblocks(nface(c, dir)), kBlock, k_deposit_faces_idx<R>, dir, c->IM, c->JM,
+                  c->KM, c->cell_start.as<int32_t>(), idx, c->sw.pid.as<int64_t>(), c->sw.w.as<R>(),
+                  c->sw.vel.as<R>(), c->face[dir].as<R>());
+      }
+      c->have_faces = true;
+    }
+    return PICGPU_OK;
+  });
+  PG_CHECK(r);
+  c->steps_since_compact++;
+  if (last || c->steps_since_compact >= kCompactEvery) {
+    PG_CHECK(compact_by_index(c, live));
+    c->steps_since_compact = 0;
+    // cell_start now indexes the compacted swarm directly (identity index)
+    c->sorted = true;
+  } else {
+    c->sorted = false;
+  }
+  return PICGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
 // stage-wise C-ABI
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -800,7 +922,15 @@ int picgpu_step(picgpu_ctx *c, double dt, int nsteps, unsigned flags) {
     set_error("ucat not uploaded");
     return PICGPU_E_STATE;
   }
+  c->steps_since_compact = 0;
   for (int s = 0; s < nsteps; ++s) {
+    if (!(flags & PICGPU_STEP_STAGED)) {
+      c->stats.particle_steps += (s == 0) ? c->n : c->n_live;
+      PG_CHECK(do_center_to_corner(c));
+      PG_CHECK(do_step_fused(c, dt, flags, s == nsteps - 1));
+      c->n_live = c->stats.located;
+      continue;
+    }
     c->stats.particle_steps += c->n;
     PG_CHECK(do_center_to_corner(c));
     if (flags & PICGPU_STEP_RK2) {

@@ -117,6 +117,8 @@ struct picgpu_ctx {
   picgpu::DevBuf guess;     // int32 [n] MLP start cell
   picgpu::DevBuf pos0, vel0;// RK2 saved state
   bool sorted = false;      // particles sorted by cell, cell_start valid
+  int64_t n_live = 0;       // particles not removed (equals n outside picgpu_step)
+  int steps_since_compact = 0;
 
   // per-cell / per-face outputs
   picgpu::DevBuf cell_count;  // int32 [ncell+1]

@@ -8,12 +8,15 @@
 //     no FMA, Makefile:15-16); sqrt and '/' are the correctly rounded IEEE
 //     operations of the HIP runtime.
 //   * eval_cell_fp32    -- the same quantities in FP32 on coordinates taken
-//     relative to vertex 0, with an a-priori error bound per face.  A face
-//     whose sign, zero-band membership, orientation flip or degeneracy is not
-//     certified by the bound is recomputed with the exact FP64 code, so the
-//     classification (and hence the walk path and the owning cell) is the
+//     relative to vertex 0, with an a-priori error bound per face.  Faces
+//     whose sign, zero-band membership, orientation flip or degeneracy the
+//     bound does not certify are recomputed by one out-of-line FP64 routine
+//     (reference arithmetic, vertices re-read from memory), so the
+//     classification -- hence the walk path and the owning cell -- is the
 //     reference's, bit for bit.  Distances (and the weights made from them)
 //     are FP32.
+// Face vertex indices are compile-time constants (template parameters), so
+// the per-lane vertex array is never indexed at run time.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -23,26 +26,45 @@ namespace picgpu {
 enum : int { F_LEFT = 0, F_RIGHT = 1, F_BOTTOM = 2, F_TOP = 3, F_FRONT = 4, F_BACK = 5 };
 enum : int { ST_NEEDS = 0, ST_LOCATED = 1, ST_MIGRATING = 2, ST_LOST = 3, ST_UNINIT = 4 };
 
-// Face vertex quadruples (v1, v2, v3, v4) in the reference's evaluation order
-// BACK, FRONT, BOTTOM, TOP, LEFT, RIGHT (walkingsearch.c:303-377) and the
-// distance slot each one fills.
-__device__ __constant__ static const int kFaceVerts[6][4] = {
+// Faces in the reference's evaluation order BACK, FRONT, BOTTOM, TOP, LEFT,
+// RIGHT (walkingsearch.c:303-377): vertices (v1, v2, v3, v4) and the distance
+// slot each fills.
+template <int F>
+struct FaceDef;
+template <> struct FaceDef<0> { static constexpr int a = 0, b = 3, c = 2, d = 1, slot = F_BACK; };
+template <> struct FaceDef<1> { static constexpr int a = 4, b = 7, c = 6, d = 5, slot = F_FRONT; };
+template <> struct FaceDef<2> { static constexpr int a = 0, b = 1, c = 6, d = 7, slot = F_BOTTOM; };
+template <> struct FaceDef<3> { static constexpr int a = 3, b = 4, c = 5, d = 2, slot = F_TOP; };
+template <> struct FaceDef<4> { static constexpr int a = 0, b = 7, c = 4, d = 3, slot = F_LEFT; };
+template <> struct FaceDef<5> { static constexpr int a = 1, b = 2, c = 5, d = 6, slot = F_RIGHT; };
+
+// the same table for the out-of-line FP64 path (indexes memory, not registers)
+__device__ __constant__ static const int8_t kFaceVerts[6][4] = {
     {0, 3, 2, 1}, {4, 7, 6, 5}, {0, 1, 6, 7}, {3, 4, 5, 2}, {0, 7, 4, 3}, {1, 2, 5, 6}};
-__device__ __constant__ static const int kFaceSlot[6] = {F_BACK, F_FRONT, F_BOTTOM, F_TOP, F_LEFT, F_RIGHT};
+__device__ __constant__ static const int8_t kFaceSlot[6] = {F_BACK, F_FRONT, F_BOTTOM, F_TOP, F_LEFT, F_RIGHT};
 
 struct GridView {
   const double *coor;  // [(KM+1)(JM+1)(IM+1)][3]
   int IM, JM, KM;
 };
 
-// vertex order V0..V7 (walkingsearch.c:537-544)
+// node index of vertex v of cell (i,j,k) (vertex order walkingsearch.c:537-544)
+__device__ __forceinline__ int64_t vertex_node(const GridView &g, int64_t base, int v) {
+  const int64_t sy = g.IM + 1, sz = (int64_t)(g.JM + 1) * (g.IM + 1);
+  const int dx = (v == 1 || v == 2 || v == 5 || v == 6) ? 1 : 0;
+  const int dy = (v == 2 || v == 3 || v == 4 || v == 5) ? 1 : 0;
+  const int dz = (v >= 4) ? 1 : 0;
+  return base + dx + dy * sy + dz * sz;
+}
+__device__ __forceinline__ int64_t cell_base_node(const GridView &g, int i, int j, int k) {
+  return (int64_t)k * (g.JM + 1) * (g.IM + 1) + (int64_t)j * (g.IM + 1) + i;
+}
+
 __device__ __forceinline__ void load_vertices(const GridView &g, int i, int j, int k, double V[8][3]) {
-  const int64_t sx = 1, sy = g.IM + 1, sz = (int64_t)(g.JM + 1) * (g.IM + 1);
-  const int64_t b = (int64_t)k * sz + (int64_t)j * sy + i;
-  const int64_t idx[8] = {b, b + sx, b + sx + sy, b + sy, b + sy + sz, b + sx + sy + sz, b + sx + sz, b + sz};
+  const int64_t b = cell_base_node(g, i, j, k);
 #pragma unroll
   for (int v = 0; v < 8; ++v) {
-    const double *q = g.coor + 3 * idx[v];
+    const double *q = g.coor + 3 * vertex_node(g, b, v);
     V[v][0] = q[0];
     V[v][1] = q[1];
     V[v][2] = q[2];
@@ -51,8 +73,8 @@ __device__ __forceinline__ void load_vertices(const GridView &g, int i, int j, i
 
 // walkingsearch.c:25-59
 __device__ __forceinline__ double char_size_exact(const double V[8][3]) {
-  const int E[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
-                        {6, 7}, {7, 4}, {0, 7}, {1, 6}, {2, 5}, {3, 4}};
+  constexpr int E[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                            {6, 7}, {7, 4}, {0, 7}, {1, 6}, {2, 5}, {3, 4}};
   double total = 0.0;
 #pragma unroll
   for (int e = 0; e < 12; ++e) {
@@ -111,18 +133,22 @@ __device__ __forceinline__ bool face_distance_exact(const double *v1, const doub
   return true;
 }
 
-// walkingsearch.c:811-859 with :271-389.  Returns false on a degenerate face.
+template <int F>
+__device__ __forceinline__ bool face_exact(const double V[8][3], const double cc[3], const double p[3], double thr,
+                                           double d[6]) {
+  using D = FaceDef<F>;
+  return face_distance_exact(V[D::a], V[D::b], V[D::c], V[D::d], cc, p, thr, d[D::slot]);
+}
+
+// walkingsearch.c:811-859 with :271-389.  Returns false on a degenerate face
+// (the reference raises at the first degenerate face in evaluation order).
 __device__ __forceinline__ bool eval_cell_exact(const double V[8][3], const double p[3], double thresh_scale,
                                                 double d[6]) {
   const double thr = thresh_scale * char_size_exact(V);
   double cc[3];
   centroid_exact(V, cc);
-#pragma unroll
-  for (int f = 0; f < 6; ++f) {
-    const int *fv = kFaceVerts[f];
-    if (!face_distance_exact(V[fv[0]], V[fv[1]], V[fv[2]], V[fv[3]], cc, p, thr, d[kFaceSlot[f]])) return false;
-  }
-  return true;
+  return face_exact<0>(V, cc, p, thr, d) && face_exact<1>(V, cc, p, thr, d) && face_exact<2>(V, cc, p, thr, d) &&
+         face_exact<3>(V, cc, p, thr, d) && face_exact<4>(V, cc, p, thr, d) && face_exact<5>(V, cc, p, thr, d);
 }
 
 // walkingsearch.c:407-462
@@ -181,90 +207,141 @@ __device__ __forceinline__ void weights_from_distances(R d[6], R clamp, R w[3]) 
 // where d, dot, m are the exact values on the FP64 inputs; the FP64
 // reference values differ from those by ~2^-29 of the same bounds.  The
 // constants below (1024, 16, 1024, 128) keep a margin over the analysis.
-// A face is certified when the decision the reference makes on it is
-// outside its bound; otherwise it is recomputed with face_distance_exact.
+// Every edge has components <= 2 Lc, so the reference's zero band
+// thr = 1e-11 * h is below 1e-11 * 6 Lc.  A face is certified when the
+// decision the reference makes on it lies outside its bound.
 // ---------------------------------------------------------------------------
+template <int F>
+__device__ __forceinline__ bool face_fp32(const float r[8][3], const float cc[3], const float q[3], float Lc2,
+                                          float Lc3, float Lw, float thrU, float d[6]) {
+  using D = FaceDef<F>;
+  const float *v1 = r[D::a], *v2 = r[D::b], *v3 = r[D::c], *v4 = r[D::d];
+  const float u = 5.9604645e-08f;  // 2^-24
+  float e1x = v2[0] - v1[0], e1y = v2[1] - v1[1], e1z = v2[2] - v1[2];
+  float e2x = v4[0] - v1[0], e2y = v4[1] - v1[1], e2z = v4[2] - v1[2];
+  float nx = __builtin_fmaf(e1y, e2z, -e1z * e2y);
+  float ny = __builtin_fmaf(e1z, e2x, -e1x * e2z);
+  float nz = __builtin_fmaf(e1x, e2y, -e1y * e2x);
+  float m = sqrtf(__builtin_fmaf(nx, nx, __builtin_fmaf(ny, ny, nz * nz)));
+  float fcx = 0.25f * ((v1[0] + v2[0]) + (v3[0] + v4[0]));
+  float fcy = 0.25f * ((v1[1] + v2[1]) + (v3[1] + v4[1]));
+  float fcz = 0.25f * ((v1[2] + v2[2]) + (v3[2] + v4[2]));
+  float dot = __builtin_fmaf(nx, cc[0] - fcx, __builtin_fmaf(ny, cc[1] - fcy, nz * (cc[2] - fcz)));
+  float dn = __builtin_fmaf(nx, fcx - q[0], __builtin_fmaf(ny, fcy - q[1], nz * (fcz - q[2])));
+  const float Em = u * (128.f * Lc2 + 2.f * m);
+  const float Edot = u * 1024.f * Lc3;
+  const float flip = (dot > 1.0e-9f) ? -1.f : 1.f;
+  const float dd = flip * (dn / m);
+  const float Ed = u * Lw * (1024.f * Lc2 / m + 16.f);
+  d[D::slot] = dd;
+  return (m - Em > 1.0e-12f * 1.001f) && (fabsf(dot - 1.0e-9f) > Edot + 1.0e-15f) && (fabsf(dd) > thrU + Ed);
+}
+
+// Out-of-line FP64 recomputation (reference arithmetic, vertices re-read
+// from memory) for the rare uncertain faces.  Arguments and results are
+// passed by value so that the caller's arrays stay in registers.
+struct ExactCell {
+  double thr, c0, c1, c2;  // zero band 1e-11*h and cell centroid
+};
+__device__ __noinline__ ExactCell exact_cell_info(const GridView g, int i, int j, int k, double thresh_scale) {
+  double V[8][3];
+  load_vertices(g, i, j, k, V);
+  ExactCell e;
+  e.thr = thresh_scale * char_size_exact(V);
+  double cc[3];
+  centroid_exact(V, cc);
+  e.c0 = cc[0];
+  e.c1 = cc[1];
+  e.c2 = cc[2];
+  return e;
+}
+struct ExactFace {
+  double d;
+  int ok;  // 0: degenerate face
+};
+__device__ __noinline__ ExactFace exact_face(const double *coor, int IM, int JM, int64_t base, int f, double p0,
+                                             double p1, double p2, double thr, double c0, double c1, double c2) {
+  const GridView g{coor, IM, JM, 0};
+  const double *v1 = coor + 3 * vertex_node(g, base, kFaceVerts[f][0]);
+  const double *v2 = coor + 3 * vertex_node(g, base, kFaceVerts[f][1]);
+  const double *v3 = coor + 3 * vertex_node(g, base, kFaceVerts[f][2]);
+  const double *v4 = coor + 3 * vertex_node(g, base, kFaceVerts[f][3]);
+  const double cc[3] = {c0, c1, c2};
+  const double p[3] = {p0, p1, p2};
+  ExactFace r;
+  r.d = 0.0;
+  r.ok = face_distance_exact(v1, v2, v3, v4, cc, p, thr, r.d) ? 1 : 0;
+  return r;
+}
+
+// store into slot kFaceSlot[f] with compile-time indices
+__device__ __forceinline__ void set_face_slot(float d[6], int f, float v) {
+  switch (f) {
+    case 0: d[F_BACK] = v; break;
+    case 1: d[F_FRONT] = v; break;
+    case 2: d[F_BOTTOM] = v; break;
+    case 3: d[F_TOP] = v; break;
+    case 4: d[F_LEFT] = v; break;
+    default: d[F_RIGHT] = v; break;
+  }
+}
+
 struct FilterResult {
   int fallbacks;  // faces recomputed in FP64
   bool degenerate;
 };
 
-__device__ __forceinline__ FilterResult eval_cell_fp32(const double V[8][3], const double p[3], double thresh_scale,
-                                                       float d[6]) {
+__device__ __forceinline__ FilterResult eval_cell_fp32(const GridView &g, int i, int j, int k, const double p[3],
+                                                       double thresh_scale, float d[6]) {
   FilterResult res{0, false};
   float r[8][3];
+  float q[3];
+  {
+    double V[8][3];
+    load_vertices(g, i, j, k, V);
+#pragma unroll
+    for (int v = 0; v < 8; ++v)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) r[v][c] = (float)(V[v][c] - V[0][c]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) q[c] = (float)(p[c] - V[0][c]);
+  }
   float Lc = 0.f;
 #pragma unroll
-  for (int v = 0; v < 8; ++v) {
+  for (int v = 1; v < 8; ++v)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      r[v][c] = (float)(V[v][c] - V[0][c]);
-      Lc = fmaxf(Lc, fabsf(r[v][c]));
-    }
-  }
-  float q[3];
-  float Lw = Lc;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    q[c] = (float)(p[c] - V[0][c]);
-    Lw = fmaxf(Lw, fabsf(q[c]));
-  }
-  // Upper bound of the zero band thr = scale * h: every edge has components
-  // <= 2 Lc, so its length (and the mean h) is <= 2*sqrt(3)*Lc < 6 Lc.
+    for (int c = 0; c < 3; ++c) Lc = fmaxf(Lc, fabsf(r[v][c]));
+  const float Lw = fmaxf(Lc, fmaxf(fabsf(q[0]), fmaxf(fabsf(q[1]), fabsf(q[2]))));
+  const float Lc2 = Lc * Lc, Lc3 = Lc2 * Lc;
   const float thrU = (float)thresh_scale * 6.0f * Lc * 1.001f + 1e-30f;
   float cc[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
     cc[c] = (((r[0][c] + r[1][c]) + (r[2][c] + r[3][c])) + ((r[4][c] + r[5][c]) + (r[6][c] + r[7][c]))) * 0.125f;
-  const float u = 5.9604645e-08f;  // 2^-24
-  const float Lc2 = Lc * Lc;
-  // exact-path state, computed on first need
-  bool have_exact = false;
-  double thr64 = 0.0, cc64[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-  for (int f = 0; f < 6; ++f) {
-    const int *fv = kFaceVerts[f];
-    const float *v1 = r[fv[0]], *v2 = r[fv[1]], *v3 = r[fv[2]], *v4 = r[fv[3]];
-    float e1x = v2[0] - v1[0], e1y = v2[1] - v1[1], e1z = v2[2] - v1[2];
-    float e2x = v4[0] - v1[0], e2y = v4[1] - v1[1], e2z = v4[2] - v1[2];
-    float nx = __builtin_fmaf(e1y, e2z, -e1z * e2y);
-    float ny = __builtin_fmaf(e1z, e2x, -e1x * e2z);
-    float nz = __builtin_fmaf(e1x, e2y, -e1y * e2x);
-    float m = sqrtf(__builtin_fmaf(nx, nx, __builtin_fmaf(ny, ny, nz * nz)));
-    float fcx = 0.25f * ((v1[0] + v2[0]) + (v3[0] + v4[0]));
-    float fcy = 0.25f * ((v1[1] + v2[1]) + (v3[1] + v4[1]));
-    float fcz = 0.25f * ((v1[2] + v2[2]) + (v3[2] + v4[2]));
-    float dot = __builtin_fmaf(nx, cc[0] - fcx, __builtin_fmaf(ny, cc[1] - fcy, nz * (cc[2] - fcz)));
-    float dn = __builtin_fmaf(nx, fcx - q[0], __builtin_fmaf(ny, fcy - q[1], nz * (fcz - q[2])));
-    const float Em = u * (128.f * Lc2 + 2.f * m);
-    const float Edot = u * 1024.f * Lc2 * Lc;
-    bool ok = (m - Em > 1.0e-12f * 1.001f);
-    ok = ok && fabsf(dot - 1.0e-9f) > Edot + 1.0e-9f * 1e-6f;
-    float dd = 0.f;
-    if (ok) {
-      const float flip = (dot > 1.0e-9f) ? -1.f : 1.f;
-      dd = flip * dn / m;
-      const float Ed = u * Lw * (1024.f * Lc2 / m + 16.f);
-      ok = fabsf(dd) > thrU + Ed;
-    }
-    if (!ok) {
-      // recompute this face with the reference arithmetic
-      if (!have_exact) {
-        have_exact = true;
-        thr64 = thresh_scale * char_size_exact(V);
-        centroid_exact(V, cc64);
-      }
-      double de = 0.0;
-      if (!face_distance_exact(V[fv[0]], V[fv[1]], V[fv[2]], V[fv[3]], cc64, p, thr64, de)) {
+  unsigned need = 0;
+  need |= face_fp32<0>(r, cc, q, Lc2, Lc3, Lw, thrU, d) ? 0u : 1u;
+  need |= face_fp32<1>(r, cc, q, Lc2, Lc3, Lw, thrU, d) ? 0u : 2u;
+  need |= face_fp32<2>(r, cc, q, Lc2, Lc3, Lw, thrU, d) ? 0u : 4u;
+  need |= face_fp32<3>(r, cc, q, Lc2, Lc3, Lw, thrU, d) ? 0u : 8u;
+  need |= face_fp32<4>(r, cc, q, Lc2, Lc3, Lw, thrU, d) ? 0u : 16u;
+  need |= face_fp32<5>(r, cc, q, Lc2, Lc3, Lw, thrU, d) ? 0u : 32u;
+  if (need) {
+    const ExactCell e = exact_cell_info(g, i, j, k, thresh_scale);
+    const int64_t base = cell_base_node(g, i, j, k);
+#pragma unroll 1
+    for (int f = 0; f < 6; ++f) {
+      if (!((need >> f) & 1u)) continue;
+      const ExactFace ef = exact_face(g.coor, g.IM, g.JM, base, f, p[0], p[1], p[2], e.thr, e.c0, e.c1, e.c2);
+      if (!ef.ok) {
         res.degenerate = true;
         return res;
       }
-      dd = (float)de;
+      float df = (float)ef.d;
       // keep the exact classification: a nonzero FP64 distance never maps to 0
-      if (de != 0.0 && dd == 0.f) dd = de < 0.0 ? -1e-30f : 1e-30f;
+      if (ef.d != 0.0 && df == 0.f) df = ef.d < 0.0 ? -1e-30f : 1e-30f;
+      set_face_slot(d, f, df);
       res.fallbacks++;
     }
-    d[kFaceSlot[f]] = dd;
   }
   return res;
 }

@@ -165,9 +165,9 @@ class PicGpu:
         L.check(self.lib.picgpu_deposit_faces(self.h, particle_volume))
 
     def AdvanceSimulation(self, dt: float, nsteps: int, rk2: bool = False, deposit_cells: bool = True,
-                          deposit_faces: bool = False):
+                          deposit_faces: bool = False, staged: bool = False):
         flags = (L.STEP_RK2 if rk2 else 0) | (L.STEP_DEPOSIT_CELLS if deposit_cells else 0) | \
-                (L.STEP_DEPOSIT_FACES if deposit_faces else 0)
+                (L.STEP_DEPOSIT_FACES if deposit_faces else 0) | (L.STEP_STAGED if staged else 0)
         L.check(self.lib.picgpu_step(self.h, dt, nsteps, flags))
 
     def synchronize(self):

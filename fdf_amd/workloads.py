@@ -303,41 +303,47 @@ def particles_uniform_tube(n, seed, R=TUBE_R, a=TUBE_A, sweep=TUBE_SWEEP, rho_ma
 
 # ---------------------------------------------------------------- configurations
 def config(c: int, scale: float = 1.0, xp=None, with_particles: bool = True) -> dict:
-    """Inputs of configuration c.  `scale` < 1 shrinks grid and particle count
-    (for tests); the full configuration is scale = 1."""
+    """Inputs of configuration c.  `scale` < 1 multiplies the number of cells
+    and of particles by `scale` (each grid axis by scale**(1/3)), keeping the
+    particles-per-cell density of the full configuration (scale = 1)."""
     xp = xp or _NumpyNS()
+    axis = scale ** (1.0 / 3.0)
 
-    def sz(v, mult=1):
-        return max(mult, int(round(v * scale / mult)) * mult)
+    def sz(v, mult=1, f=None):
+        f = axis if f is None else f
+        return max(mult, int(round(v * f / mult)) * mult)
+
+    def npart(v):
+        return max(1, int(round(v * scale)))
 
     d: dict = {"config": c}
     if c == 1:
         IM, JM, KM = 20, 20, 280
         coor = uniform_grid(IM, JM, KM, xp=xp)
         ucat = ucat_uniform(IM, JM, KM, (0.0, 0.0, 1.0), xp=xp)
-        n = sz(10_000)
+        n = npart(10_000)
         d.update(dt=0.1, steps=10, rk2=False, deposit="cells", particles="cube")
     elif c in (2, 4):
         IM, JM, KM = (64, 64, 512) if c == 2 else (128, 128, 1024)
         IM, JM, KM = sz(IM, 2), sz(JM, 2), sz(KM)
         coor = tube_grid(IM, JM, KM, xp=xp)
         ucat = ucat_poiseuille(coor, IM, JM, KM, seed=2000 + c, xp=xp)
-        n = sz(4_194_304) if c == 2 else sz(16_777_216)
+        n = npart(4_194_304) if c == 2 else npart(16_777_216)
         d.update(dt=0.5 * (TUBE_R * TUBE_SWEEP / KM) / 1.0, steps=50 if c == 2 else 10, rk2=(c == 2),
                  deposit="faces_volfrac" if c == 2 else "none", particles="tube")
         if c == 4:
-            d["train_pairs"] = sz(2_097_152)
+            d["train_pairs"] = npart(2_097_152)
     elif c in (3, 5):
         IM, JM, KM = (128, 128, 1024) if c == 3 else (256, 256, 1024)
         IM, JM, KM = sz(IM), sz(JM), sz(KM)
         coor = perturbed_cube(IM, JM, KM, seed=3000 + c, xp=xp)
         if c == 3:
             ucat = ucat_taylor_green(coor, IM, JM, KM, xp=xp)
-            n = sz(33_554_432)
+            n = npart(33_554_432)
             d.update(dt=0.5 / KM, steps=20, rk2=False, deposit="cells+faces", particles="cube")
         else:
             ucat = ucat_random_fourier(coor, IM, JM, KM, seed=2000 + c, xp=xp)
-            n = sz(134_217_728)
+            n = npart(134_217_728)
             umax = float(abs(ucat).max())
             d.update(dt=0.5 * (1.0 / KM) / umax, steps=100, rk2=True, deposit="faces", particles="cube")
     else:

@@ -234,9 +234,10 @@ def _run_oracle_steps(oracle, d, nsteps, rk2):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("path", ["fused", "staged"])
 @pytest.mark.parametrize("case", ["config1", "tube_rk2", "warped_rk2"])
-def test_full_steps_against_oracle(swarm_cls, oracle, case, precision):
-    _say("full_steps", case, precision)
+def test_full_steps_against_oracle(swarm_cls, oracle, case, precision, path):
+    _say("full_steps", case, precision, path)
     if case == "config1":
         d = W.config(1)
         nsteps, rk2 = (10 if precision == "fp64" else 5), False
@@ -256,10 +257,16 @@ def test_full_steps_against_oracle(swarm_cls, oracle, case, precision):
         g.upload_particles(d["pos"])
         g.LocateAllParticlesInGrid()
         g.InterpolateAllFieldsToSwarm()
-        g.AdvanceSimulation(d["dt"], nsteps, rk2=rk2, deposit_cells=True, deposit_faces=True)
+        # split the steps over two calls: the fused path compacts between calls
+        g.AdvanceSimulation(d["dt"], nsteps // 2, rk2=rk2, deposit_cells=True, deposit_faces=True,
+                            staged=path == "staged")
+        g.AdvanceSimulation(d["dt"], nsteps - nsteps // 2, rk2=rk2, deposit_cells=True, deposit_faces=True,
+                            staged=path == "staged")
         out = _by_pid(g.download_particles())
         cnt = g.field(L.FIELD_PARTICLE_COUNT)
+        pav = g.field(L.FIELD_P_AVERAGE)
         faces = [g.field(L.FIELD_IFACE), g.field(L.FIELD_JFACE), g.field(L.FIELD_KFACE)]
+    assert set(np.unique(pav)) <= {0.0, 101325.0} and ((pav > 0) == (cnt > 0)).all()
     if case == "config1" and nsteps == 10:
         # z moves by 10 x 0.1: every particle leaves the unit cube (simulation.c:263-265 removes them)
         assert len(s["pid"]) <= 1
@@ -290,3 +297,28 @@ def test_full_steps_against_oracle(swarm_cls, oracle, case, precision):
         v2 = oracle.interp(IM, JM, KM, corner, c2, w2)
         assert np.abs(v2 - out["vel"]).max() <= FP32_TOL * np.abs(d["ucat"]).max()
         assert np.array_equal(cnt, oracle.count_cells(IM, JM, KM, out["cell"]))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_long_fused_run_matches_staged(swarm_cls, precision):
+    """20 steps in one call (the fused path compacts the swarm mid-call) give
+    the same particles as the stage-by-stage path."""
+    _say("long_fused_vs_staged", precision)
+    d = W.config(5, scale=0.02)
+    d["pos"] = d["pos"][:60000]
+    outs = []
+    for staged in (False, True):
+        with swarm_cls(precision=precision) as g:
+            g.upload_grid(d["IM"], d["JM"], d["KM"], d["coor"])
+            g.upload_ucat(d["ucat"])
+            g.upload_particles(d["pos"])
+            g.LocateAllParticlesInGrid()
+            g.InterpolateAllFieldsToSwarm()
+            g.AdvanceSimulation(d["dt"] * 4, 20, rk2=True, deposit_cells=True, deposit_faces=True, staged=staged)
+            outs.append((_by_pid(g.download_particles()), g.field(L.FIELD_KFACE), g.stats()))
+    (a, fa, sa), (b, fb, sb) = outs
+    for k in ("pid", "cell", "pos", "vel", "weight", "status"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(fa, fb)
+    assert sa["particle_steps"] == sb["particle_steps"]
+    assert len(a["pid"]) < 60000  # some particles left the domain
