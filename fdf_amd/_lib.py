@@ -108,6 +108,7 @@ _SIG = {
     "picgpu_mlp_forward": (C.c_int, [_P, _I64, _P, _P]),
     "picgpu_mlp_loss_grad": (C.c_int, [_P, _I64, _P, _P, C.POINTER(C.c_float), _P]),
     "picgpu_mlp_guess": (C.c_int, [_P, _I64, _P, _P]),
+    "picgpu_mlp_enable": (C.c_int, [_P, C.c_int]),
 }
 
 _lib = None

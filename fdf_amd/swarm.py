@@ -173,6 +173,49 @@ class PicGpu:
     def synchronize(self):
         L.check(self.lib.picgpu_synchronize(self.h))
 
+    # ---- MLP initial guess ----
+    def mlp_param_count(self) -> int:
+        return int(self.lib.picgpu_mlp_param_count())
+
+    def mlp_set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == self.mlp_param_count()
+        L.check(self.lib.picgpu_mlp_set_params(self.h, _ptr(p)))
+
+    def mlp_get_params(self) -> np.ndarray:
+        p = np.empty(self.mlp_param_count(), np.float32)
+        L.check(self.lib.picgpu_mlp_get_params(self.h, _ptr(p)))
+        return p
+
+    def mlp_forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+        y = np.empty_like(x)
+        L.check(self.lib.picgpu_mlp_forward(self.h, x.shape[0], _ptr(x), _ptr(y)))
+        return y
+
+    def mlp_loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, 3)
+        g = np.empty(self.mlp_param_count(), np.float32)
+        loss = C.c_float()
+        L.check(self.lib.picgpu_mlp_loss_grad(self.h, x.shape[0], _ptr(x), _ptr(t), C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def mlp_train(self, max_pairs: int = 2_097_152, batch: int = 4096, iters: int = 2000, lr: float = 3e-3,
+                  seed: int = 3004) -> float:
+        loss = C.c_float()
+        L.check(self.lib.picgpu_mlp_train(self.h, max_pairs, batch, iters, lr, seed, C.byref(loss)))
+        return loss.value
+
+    def mlp_guess(self, pos) -> np.ndarray:
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        cell = np.empty((pos.shape[0], 3), np.int64)
+        L.check(self.lib.picgpu_mlp_guess(self.h, pos.shape[0], _ptr(pos), _ptr(cell)))
+        return cell
+
+    def mlp_enable(self, on: bool = True):
+        L.check(self.lib.picgpu_mlp_enable(self.h, int(on)))
+
     # ---- timing ----
     def timing(self, on: bool = True):
         L.check(self.lib.picgpu_timing_enable(self.h, int(on)))

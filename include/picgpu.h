@@ -179,6 +179,8 @@ int picgpu_mlp_loss_grad(picgpu_ctx *ctx, int64_t n, const float *x, const float
                          float *grad);
 /* start cells the MLP proposes for positions pos [n][3]: cell [n][3] */
 int picgpu_mlp_guess(picgpu_ctx *ctx, int64_t n, const double *pos, int64_t *cell);
+/* use (1) or not (0) the MLP start cell for particles without a valid cell */
+int picgpu_mlp_enable(picgpu_ctx *ctx, int on);
 
 #ifdef __cplusplus
 }
