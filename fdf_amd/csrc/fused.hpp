@@ -1,15 +1,21 @@
-// fused.hpp -- one-pass particle step for gfx950.
+// fused.hpp -- one-pass particle step for gfx950, tile-staged.
 //
-// k_step_fused does, per particle and in one pass over the swarm, the stages
+// k_step_tiled does, per particle and in one pass over the swarm, the stages
 // the reference runs as separate sweeps (simulation.c:248-292): status reset,
 // advection (forward Euler, ParticleMotion.c:19-29, or the Heun predictor /
 // corrector), the walking search from the previous cell after each position
 // update (walkingsearch.c:1094-1242), removal of LOST and out-of-box
 // particles (ParticleMotion.c:532-625, 283-383) and trilinear interpolation
-// of the corner velocity (interpolation.c:957-1042).  It ends by taking a
-// slot in its cell's histogram bucket, from which a counting sort builds the
-// cell-ordered index used by deposition.  Removed particles stay in the
-// arrays, marked ST_REMOVED, until the swarm is compacted.
+// of the corner velocity (interpolation.c:957-1042).
+//
+// Cells are grouped in 8x8x4 tiles; particles are kept in tile-major cell
+// order (a counting sort by the tile-major cell key after every step).  One
+// workgroup processes the particles of one tile and first stages the
+// tile's node coordinates and corner velocities, with a one-cell halo, in
+// LDS; walks that stay within the halo read LDS only.  Each particle ends by
+// taking a slot in the bucket of its new key, from which the next step's
+// index is built.  Removed particles stay in the arrays, marked ST_REMOVED,
+// until the swarm is compacted.
 #pragma once
 #include "kernels.hpp"
 
@@ -17,10 +23,23 @@ namespace picgpu {
 
 constexpr int ST_REMOVED = 5;  // internal: removed, awaiting compaction
 
+// tile geometry
+constexpr int TX = 8, TY = 8, TZ = 4, TS = TX * TY * TZ;  // cells per tile
+constexpr int BX = TX + 3, BY = TY + 3, BZ = TZ + 3, BN = BX * BY * BZ;  // staged nodes (one-cell halo)
+
+struct TileMap {
+  int NTI, NTJ, NTK;  // tiles per axis
+};
+
+__device__ __forceinline__ int32_t cell_key(const TileMap &t, int i, int j, int k) {
+  const int ti = i / TX, tj = j / TY, tk = k / TZ;
+  const int li = i - ti * TX, lj = j - tj * TY, lk = k - tk * TZ;
+  return (((tk * t.NTJ + tj) * t.NTI + ti) * TS) + ((lk * TY + lj) * TX + li);
+}
+
 template <typename R>
 struct StepArgs {
   GridView g;
-  int64_t n;
   double *pos;
   R *vel;
   R *w;
@@ -32,20 +51,119 @@ struct StepArgs {
   double thresh_scale;
   R clamp;
   double bx0, by0, bz0, bx1, by1, bz1;
-  int32_t *count;  // [ncell] histogram (zeroed by the caller)
-  int32_t *slot;   // [n] slot in the cell bucket, -1 if removed
+  TileMap tm;
+  int32_t *count_next;  // [nkeys] histogram of the new keys (zeroed by the caller)
+  int32_t *slot;        // [n] slot in the new key bucket, -1 if removed
   unsigned long long *counters;
 };
 
 enum WalkOutcome : int { W_LOCATED = 0, W_LOST = 1, W_DEGENERATE = 2 };
 
+// corner velocities of cell (i,j,k) in the order of interpolation.c:990-999
+template <typename R>
+__device__ __forceinline__ void trilinear_vals(const R C[8][3], R a1, R a2, R a3, R out[3]) {
+  a1 = ((R)0 < ((R)1 < a1 ? (R)1 : a1)) ? ((R)1 < a1 ? (R)1 : a1) : (R)0;
+  a2 = ((R)0 < ((R)1 < a2 ? (R)1 : a2)) ? ((R)1 < a2 ? (R)1 : a2) : (R)0;
+  a3 = ((R)0 < ((R)1 < a3 ? (R)1 : a3)) ? ((R)1 < a3 ? (R)1 : a3) : (R)0;
+  const R oa1 = (R)1 - a1, oa2 = (R)1 - a2, oa3 = (R)1 - a3;
+  R wc[8];
+  wc[0] = oa1 * oa2 * oa3;
+  wc[1] = a1 * oa2 * oa3;
+  wc[2] = oa1 * a2 * oa3;
+  wc[3] = a1 * a2 * oa3;
+  wc[4] = oa1 * oa2 * a3;
+  wc[5] = a1 * oa2 * a3;
+  wc[6] = oa1 * a2 * a3;
+  wc[7] = a1 * a2 * a3;
+  R sw = 0, ax = 0, ay = 0, az = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    ax += wc[c] * C[c][0];
+    ay += wc[c] * C[c][1];
+    az += wc[c] * C[c][2];
+    sw += wc[c];
+  }
+  if (sw > (R)1.0e-14) {
+    out[0] = ax / sw;
+    out[1] = ay / sw;
+    out[2] = az / sw;
+  } else {
+    out[0] = out[1] = out[2] = (R)0;
+  }
+}
+
+// Grid data straight from global memory.
+template <typename R>
+struct GlobalSrc {
+  GridView g;
+  const R *corner;
+  __device__ __forceinline__ void vertices(int i, int j, int k, double V[8][3]) const { load_vertices(g, i, j, k, V); }
+  __device__ __forceinline__ void corners(int i, int j, int k, R C[8][3]) const {
+    const int64_t sy = g.IM + 1, sz = (int64_t)(g.JM + 1) * (g.IM + 1);
+    const int64_t b = (int64_t)k * sz + (int64_t)j * sy + i;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int64_t q = b + (c & 1) + ((c >> 1) & 1) * sy + ((c >> 2) & 1) * sz;
+      C[c][0] = corner[3 * q];
+      C[c][1] = corner[3 * q + 1];
+      C[c][2] = corner[3 * q + 2];
+    }
+  }
+};
+
+// Grid data of a tile (plus halo) staged in LDS; other cells from global memory.
+template <typename R>
+struct TileSrc {
+  GridView g;
+  const R *corner;
+  const double *sc;  // [BZ][BY][BX][3]
+  const R *sk;       // [BZ][BY][BX][3]
+  int i0, j0, k0;    // node origin of the staged box
+  __device__ __forceinline__ bool staged(int i, int j, int k) const {
+    return (unsigned)(i - i0) < (unsigned)(BX - 1) && (unsigned)(j - j0) < (unsigned)(BY - 1) &&
+           (unsigned)(k - k0) < (unsigned)(BZ - 1);
+  }
+  __device__ __forceinline__ void vertices(int i, int j, int k, double V[8][3]) const {
+    if (staged(i, j, k)) {
+      const int b = ((k - k0) * BY + (j - j0)) * BX + (i - i0);
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        const int dx = (v == 1 || v == 2 || v == 5 || v == 6) ? 1 : 0;
+        const int dy = (v == 2 || v == 3 || v == 4 || v == 5) ? 1 : 0;
+        const int dz = (v >= 4) ? 1 : 0;
+        const double *q = sc + 3 * (b + dx + dy * BX + dz * BX * BY);
+        V[v][0] = q[0];
+        V[v][1] = q[1];
+        V[v][2] = q[2];
+      }
+    } else {
+      load_vertices(g, i, j, k, V);
+    }
+  }
+  __device__ __forceinline__ void corners(int i, int j, int k, R C[8][3]) const {
+    if (staged(i, j, k)) {
+      const int b = ((k - k0) * BY + (j - j0)) * BX + (i - i0);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const R *q = sk + 3 * (b + (c & 1) + ((c >> 1) & 1) * BX + ((c >> 2) & 1) * BX * BY);
+        C[c][0] = q[0];
+        C[c][1] = q[1];
+        C[c][2] = q[2];
+      }
+    } else {
+      GlobalSrc<R>{g, corner}.corners(i, j, k, C);
+    }
+  }
+};
+
 // The walk of walkingsearch.c:1123-1212 from cell (ci,cj,ck) (negative ->
 // (0,0,0), :611-616) for one lane.  On success (ci,cj,ck) is the owning cell
 // and wv the weights.
-template <typename R, bool EXACT>
-__device__ __forceinline__ int walk_one(const GridView &g, const double x[3], int &ci, int &cj, int &ck,
+template <typename R, bool EXACT, class Src>
+__device__ __forceinline__ int walk_one(const Src &src, const double x[3], int &ci, int &cj, int &ck,
                                         int max_traversal, double thresh_scale, R clamp, R wv[3], int &steps,
                                         unsigned long long &fallbacks) {
+  const GridView &g = src.g;
   if (ci < 0 || cj < 0 || ck < 0) {
     ci = 0;
     cj = 0;
@@ -61,16 +179,16 @@ __device__ __forceinline__ int walk_one(const GridView &g, const double x[3], in
       return W_LOST;
     }
     R d[6];
+    double V[8][3];
+    src.vertices(ci, cj, ck, V);
     if constexpr (EXACT) {
-      double V[8][3];
-      load_vertices(g, ci, cj, ck, V);
       double dd[6];
       if (!eval_cell_exact(V, x, thresh_scale, dd)) return W_DEGENERATE;
 #pragma unroll
       for (int f = 0; f < 6; ++f) d[f] = dd[f];
     } else {
       float dd[6];
-      FilterResult fr = eval_cell_fp32(g, ci, cj, ck, x, thresh_scale, dd);
+      FilterResult fr = eval_cell_fp32_v(V, g, ci, cj, ck, x, thresh_scale, dd);
       if (fr.degenerate) return W_DEGENERATE;
       fallbacks += fr.fallbacks;
 #pragma unroll
@@ -85,103 +203,106 @@ __device__ __forceinline__ int walk_one(const GridView &g, const double x[3], in
   return W_LOST;
 }
 
-template <typename R>
-__device__ __forceinline__ void interp_at(const GridView &g, const R *__restrict__ corner, int ci, int cj, int ck,
-                                          const R wv[3], R u[3]) {
-  trilinear<R>(g.IM, g.JM, corner, ci, cj, ck, wv[0], wv[1], wv[2], u);
+template <typename R, class Src>
+__device__ __forceinline__ void interp_at(const Src &src, int ci, int cj, int ck, const R wv[3], R u[3]) {
+  R C[8][3];
+  src.corners(ci, cj, ck, C);
+  trilinear_vals<R>(C, wv[0], wv[1], wv[2], u);
 }
 
-template <typename R, bool EXACT, bool RK2>
-__global__ __launch_bounds__(kBlock) void k_step_fused(StepArgs<R> a) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int located = 0, lost = 0, oob = 0, err = 0, iters = 0;
-  unsigned long long fb = 0;
-  if (p < a.n) {
-    const int32_t st = a.status[p];
-    if (st == ST_REMOVED) {
-      a.slot[p] = -1;
+struct StepTally {
+  unsigned long long located = 0, lost = 0, oob = 0, iters = 0, fb = 0;
+  int err = 0;
+};
+
+// One particle's step (shared by the tiled and the generic kernel).
+template <typename R, bool EXACT, bool RK2, class Src>
+__device__ __forceinline__ void step_particle(const StepArgs<R> &a, const Src &src, int64_t p, StepTally &t) {
+  const int32_t st = a.status[p];
+  if (st == ST_REMOVED) {
+    a.slot[p] = -1;
+    return;
+  }
+  const double x0[3] = {a.pos[3 * p], a.pos[3 * p + 1], a.pos[3 * p + 2]};
+  const R u0[3] = {a.vel[3 * p], a.vel[3 * p + 1], a.vel[3 * p + 2]};
+  int ci = -1, cj = -1, ck = -1;
+  const int32_t c0 = a.cell[p];
+  if (c0 >= 0) unpack_cell(c0, a.g.IM, a.g.JM, ci, cj, ck);
+  double x1[3];
+  R wv[3] = {a.w[3 * p], a.w[3 * p + 1], a.w[3 * p + 2]};
+  int outcome;
+  int s1 = 0, s2 = 0;
+  if constexpr (RK2) {
+    if (st == ST_LOST) {
+      // LOST at step start: not advanced, removed
+      x1[0] = x0[0];
+      x1[1] = x0[1];
+      x1[2] = x0[2];
+      outcome = W_LOST;
     } else {
-      const double x0[3] = {a.pos[3 * p], a.pos[3 * p + 1], a.pos[3 * p + 2]};
-      const R u0[3] = {a.vel[3 * p], a.vel[3 * p + 1], a.vel[3 * p + 2]};
-      int ci = -1, cj = -1, ck = -1;
-      const int32_t c0 = a.cell[p];
-      if (c0 >= 0) unpack_cell(c0, a.g.IM, a.g.JM, ci, cj, ck);
-      double x1[3];
-      R wv[3] = {a.w[3 * p], a.w[3 * p + 1], a.w[3 * p + 2]};
-      int outcome;
-      int s1 = 0, s2 = 0;
-      if constexpr (RK2) {
-        if (st == ST_LOST) {
-          // LOST at step start: not advanced, removed
-          x1[0] = x0[0];
-          x1[1] = x0[1];
-          x1[2] = x0[2];
-          outcome = W_LOST;
-        } else {
-          double xs[3];
+      double xs[3];
 #pragma unroll
-          for (int c = 0; c < 3; ++c) xs[c] = x0[c] + (double)u0[c] * a.dt;
-          outcome = walk_one<R, EXACT>(a.g, xs, ci, cj, ck, a.max_traversal, a.thresh_scale, a.clamp, wv, s1, fb);
-          if (outcome == W_LOCATED) {
-            R us[3];
-            interp_at<R>(a.g, a.corner, ci, cj, ck, wv, us);
-            const double h = 0.5 * a.dt;
+      for (int c = 0; c < 3; ++c) xs[c] = x0[c] + (double)u0[c] * a.dt;
+      outcome = walk_one<R, EXACT>(src, xs, ci, cj, ck, a.max_traversal, a.thresh_scale, a.clamp, wv, s1, t.fb);
+      if (outcome == W_LOCATED) {
+        R us[3];
+        interp_at<R>(src, ci, cj, ck, wv, us);
+        const double h = 0.5 * a.dt;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) x1[c] = x0[c] + ((double)u0[c] + (double)us[c]) * h;
-            outcome =
-                walk_one<R, EXACT>(a.g, x1, ci, cj, ck, a.max_traversal, a.thresh_scale, a.clamp, wv, s2, fb);
-          } else {
-            x1[0] = xs[0];
-            x1[1] = xs[1];
-            x1[2] = xs[2];
-          }
-        }
+        for (int c = 0; c < 3; ++c) x1[c] = x0[c] + ((double)u0[c] + (double)us[c]) * h;
+        outcome = walk_one<R, EXACT>(src, x1, ci, cj, ck, a.max_traversal, a.thresh_scale, a.clamp, wv, s2, t.fb);
       } else {
-        // Euler: every particle is advanced, then located from its cell
-#pragma unroll
-        for (int c = 0; c < 3; ++c) x1[c] = x0[c] + (double)u0[c] * a.dt;
-        outcome = walk_one<R, EXACT>(a.g, x1, ci, cj, ck, a.max_traversal, a.thresh_scale, a.clamp, wv, s1, fb);
-      }
-      iters = s1 + s2;
-      a.pos[3 * p] = x1[0];
-      a.pos[3 * p + 1] = x1[1];
-      a.pos[3 * p + 2] = x1[2];
-      if (outcome == W_DEGENERATE) {
-        err = 1;
-        a.slot[p] = -1;
-      } else if (outcome == W_LOST) {
-        lost = 1;
-        a.cell[p] = -1;
-        a.status[p] = ST_REMOVED;
-        a.slot[p] = -1;
-      } else {
-        const bool in = x1[0] >= a.bx0 && x1[0] <= a.bx1 && x1[1] >= a.by0 && x1[1] <= a.by1 && x1[2] >= a.bz0 &&
-                        x1[2] <= a.bz1;
-        const int32_t c1 = pack_cell(ci, cj, ck, a.g.IM, a.g.JM);
-        a.cell[p] = c1;
-        a.w[3 * p] = wv[0];
-        a.w[3 * p + 1] = wv[1];
-        a.w[3 * p + 2] = wv[2];
-        if (in) {
-          located = 1;
-          R u1[3];
-          interp_at<R>(a.g, a.corner, ci, cj, ck, wv, u1);
-          a.vel[3 * p] = u1[0];
-          a.vel[3 * p + 1] = u1[1];
-          a.vel[3 * p + 2] = u1[2];
-          a.status[p] = ST_LOCATED;
-          a.slot[p] = atomicAdd(&a.count[c1], 1);
-        } else {
-          oob = 1;
-          a.status[p] = ST_REMOVED;
-          a.slot[p] = -1;
-        }
+        x1[0] = xs[0];
+        x1[1] = xs[1];
+        x1[2] = xs[2];
       }
     }
+  } else {
+    // Euler: every particle is advanced, then located from its cell
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x1[c] = x0[c] + (double)u0[c] * a.dt;
+    outcome = walk_one<R, EXACT>(src, x1, ci, cj, ck, a.max_traversal, a.thresh_scale, a.clamp, wv, s1, t.fb);
   }
-  // one set of atomics per wavefront
-  unsigned long long v_loc = located, v_lost = lost, v_oob = oob, v_it = iters, v_fb = fb;
-  int v_err = err;
+  t.iters += s1 + s2;
+  a.pos[3 * p] = x1[0];
+  a.pos[3 * p + 1] = x1[1];
+  a.pos[3 * p + 2] = x1[2];
+  if (outcome == W_DEGENERATE) {
+    t.err = 1;
+    a.slot[p] = -1;
+  } else if (outcome == W_LOST) {
+    t.lost++;
+    a.cell[p] = -1;
+    a.status[p] = ST_REMOVED;
+    a.slot[p] = -1;
+  } else {
+    const bool in = x1[0] >= a.bx0 && x1[0] <= a.bx1 && x1[1] >= a.by0 && x1[1] <= a.by1 && x1[2] >= a.bz0 &&
+                    x1[2] <= a.bz1;
+    a.cell[p] = pack_cell(ci, cj, ck, a.g.IM, a.g.JM);
+    a.w[3 * p] = wv[0];
+    a.w[3 * p + 1] = wv[1];
+    a.w[3 * p + 2] = wv[2];
+    if (in) {
+      t.located++;
+      R u1[3];
+      interp_at<R>(src, ci, cj, ck, wv, u1);
+      a.vel[3 * p] = u1[0];
+      a.vel[3 * p + 1] = u1[1];
+      a.vel[3 * p + 2] = u1[2];
+      a.status[p] = ST_LOCATED;
+      a.slot[p] = atomicAdd(&a.count_next[cell_key(a.tm, ci, cj, ck)], 1);
+    } else {
+      t.oob++;
+      a.status[p] = ST_REMOVED;
+      a.slot[p] = -1;
+    }
+  }
+}
+
+// one set of atomics per wavefront; every lane of the wave must call it
+__device__ __forceinline__ void flush_tally(StepTally &t, unsigned long long *counters) {
+  unsigned long long v_loc = t.located, v_lost = t.lost, v_oob = t.oob, v_it = t.iters, v_fb = t.fb;
+  int v_err = t.err;
   for (int off = 32; off > 0; off >>= 1) {
     v_loc += __shfl_xor(v_loc, off);
     v_lost += __shfl_xor(v_lost, off);
@@ -191,119 +312,244 @@ __global__ __launch_bounds__(kBlock) void k_step_fused(StepArgs<R> a) {
     v_err |= __shfl_xor(v_err, off);
   }
   if ((threadIdx.x & 63) == 0) {
-    if (v_loc) atomicAdd(&a.counters[C_LOCATED], v_loc);
-    if (v_lost) atomicAdd(&a.counters[C_REMOVED_LOST], v_lost);
-    if (v_oob) atomicAdd(&a.counters[C_REMOVED_OOB], v_oob);
-    if (v_it) atomicAdd(&a.counters[C_WALK_ITERS], v_it);
-    if (v_fb) atomicAdd(&a.counters[C_FALLBACKS], v_fb);
-    if (v_err) atomicOr(&a.counters[C_ERROR], 1ull);
+    if (v_loc) atomicAdd(&counters[C_LOCATED], v_loc);
+    if (v_lost) atomicAdd(&counters[C_REMOVED_LOST], v_lost);
+    if (v_oob) atomicAdd(&counters[C_REMOVED_OOB], v_oob);
+    if (v_it) atomicAdd(&counters[C_WALK_ITERS], v_it);
+    if (v_fb) atomicAdd(&counters[C_FALLBACKS], v_fb);
+    if (v_err) atomicOr(&counters[C_ERROR], 1ull);
   }
 }
 
-// idx[start[c] + slot[p]] = p for every live particle.
-__global__ __launch_bounds__(kBlock) void k_index_scatter(int64_t n, const int32_t *__restrict__ cell,
-                                                          const int32_t *__restrict__ slot,
-                                                          const int32_t *__restrict__ start, int32_t *__restrict__ idx) {
+// One workgroup per tile: stage the tile's grid data in LDS, then step the
+// tile's particles (idx[kstart[t*TS] .. kstart[(t+1)*TS])).
+template <typename R, bool EXACT, bool RK2>
+__global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int32_t *__restrict__ idx,
+                                                       const int32_t *__restrict__ kstart) {
+  __shared__ double sc[BN * 3];
+  __shared__ R sk[BN * 3];
+  const int t = blockIdx.x;
+  const int32_t s = kstart[(int64_t)t * TS], e = kstart[(int64_t)(t + 1) * TS];
+  if (s == e) return;  // empty tile (uniform over the block)
+  const int ti = t % a.tm.NTI, tj = (t / a.tm.NTI) % a.tm.NTJ, tk = t / (a.tm.NTI * a.tm.NTJ);
+  TileSrc<R> src{a.g, a.corner, sc, sk, ti * TX - 1, tj * TY - 1, tk * TZ - 1};
+  const int64_t sy = a.g.IM + 1, sz = (int64_t)(a.g.JM + 1) * (a.g.IM + 1);
+  for (int q = threadIdx.x; q < BN; q += blockDim.x) {
+    const int bx = q % BX, by = (q / BX) % BY, bz = q / (BX * BY);
+    const int ni = src.i0 + bx, nj = src.j0 + by, nk = src.k0 + bz;
+    if (ni >= 0 && ni <= a.g.IM && nj >= 0 && nj <= a.g.JM && nk >= 0 && nk <= a.g.KM) {
+      const int64_t node = (int64_t)nk * sz + (int64_t)nj * sy + ni;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        sc[3 * q + c] = a.g.coor[3 * node + c];
+        sk[3 * q + c] = a.corner[3 * node + c];
+      }
+    }
+  }
+  __syncthreads();
+  StepTally tally;
+  for (int32_t q = s + (int32_t)threadIdx.x; q < e; q += (int32_t)blockDim.x)
+    step_particle<R, EXACT, RK2>(a, src, (int64_t)idx[q], tally);
+  flush_tally(tally, a.counters);
+}
+
+// Particles idx[q0 .. q1) from global memory (particles without a cell).
+template <typename R, bool EXACT, bool RK2>
+__global__ __launch_bounds__(kBlock) void k_step_list(StepArgs<R> a, const int32_t *__restrict__ idx, int64_t q0,
+                                                      int64_t q1) {
+  const int64_t q = q0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  StepTally tally;
+  if (q < q1) step_particle<R, EXACT, RK2>(a, GlobalSrc<R>{a.g, a.corner}, (int64_t)idx[q], tally);
+  flush_tally(tally, a.counters);
+}
+
+// Bucket of every particle for the index: its cell's key, nkeys for a live
+// particle without a cell, none for a removed one.
+__device__ __forceinline__ int32_t bucket_of(int32_t c, int32_t st, int IM, int JM, const TileMap &tm,
+                                             int32_t nkeys) {
+  if (c >= 0) {
+    int i, j, k;
+    unpack_cell(c, IM, JM, i, j, k);
+    return cell_key(tm, i, j, k);
+  }
+  return st == ST_REMOVED ? -1 : nkeys;
+}
+
+__global__ __launch_bounds__(kBlock) void k_key_histogram(int64_t n, const int32_t *__restrict__ cell,
+                                                          const int32_t *__restrict__ status, int IM, int JM,
+                                                          TileMap tm, int32_t nkeys, int32_t *__restrict__ count,
+                                                          int32_t *__restrict__ slot) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  int32_t s = slot[p];
+  const int32_t b = bucket_of(cell[p], status[p], IM, JM, tm, nkeys);
+  slot[p] = b >= 0 ? atomicAdd(&count[b], 1) : -1;
+}
+
+// idx[kstart[bucket] + slot] = p
+__global__ __launch_bounds__(kBlock) void k_key_scatter(int64_t n, const int32_t *__restrict__ cell,
+                                                        const int32_t *__restrict__ status,
+                                                        const int32_t *__restrict__ slot, int IM, int JM, TileMap tm,
+                                                        int32_t nkeys, const int32_t *__restrict__ kstart,
+                                                        int32_t *__restrict__ idx) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int32_t s = slot[p];
   if (s < 0) return;
-  idx[start[cell[p]] + s] = (int32_t)p;
+  const int32_t b = bucket_of(cell[p], status[p], IM, JM, tm, nkeys);
+  idx[kstart[b] + s] = (int32_t)p;
 }
 
-// Visit cell segment [s,e) of idx in ascending PID order.
-__device__ __forceinline__ int64_t next_by_pid_idx(const int32_t *__restrict__ idx, const int64_t *__restrict__ pid,
-                                                   int64_t s, int64_t e, int64_t last, int64_t &out_pid) {
-  int64_t best = -1, bp = INT64_MAX;
-  for (int64_t q = s; q < e; ++q) {
-    int64_t v = pid[idx[q]];
-    if (v > last && v < bp) {
-      bp = v;
-      best = idx[q];
-    }
+// After a step: idx_next[kstart_next[key(cell[p])] + slot[p]] = p for the
+// particles p = idx_cur[0..m) that survived it (slot >= 0).
+__global__ __launch_bounds__(kBlock) void k_key_reindex(int64_t m, const int32_t *__restrict__ idx_cur,
+                                                        const int32_t *__restrict__ cell,
+                                                        const int32_t *__restrict__ slot, int IM, int JM, TileMap tm,
+                                                        const int32_t *__restrict__ kstart,
+                                                        int32_t *__restrict__ idx_next) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  const int32_t p = idx_cur[q];
+  const int32_t s = slot[p];
+  if (s < 0) return;
+  int i, j, k;
+  unpack_cell(cell[p], IM, JM, i, j, k);
+  idx_next[kstart[cell_key(tm, i, j, k)] + s] = p;
+}
+
+__global__ __launch_bounds__(kBlock) void k_iota(int64_t n, int32_t *__restrict__ idx) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) idx[q] = (int32_t)q;
+}
+
+// Calls f(p) for the particles idx[s..e) in ascending PID order (the
+// summation order of the oracle); cells of one or two particles, the common
+// case, take a direct path.
+template <class F>
+__device__ __forceinline__ void visit_by_pid(const int32_t *__restrict__ idx, const int64_t *__restrict__ pid,
+                                             int64_t s, int64_t e, F &&f) {
+  const int64_t L = e - s;
+  if (L <= 0) return;
+  if (L == 1) {
+    f((int64_t)idx[s]);
+    return;
   }
-  out_pid = bp;
-  return best;
-}
-
-// Face deposition through the cell-ordered index (same definition and
-// summation order as k_deposit_faces).
-template <typename R>
-__global__ __launch_bounds__(kBlock) void k_deposit_faces_idx(int dir, int IM, int JM, int KM,
-                                                              const int32_t *__restrict__ start,
-                                                              const int32_t *__restrict__ idx,
-                                                              const int64_t *__restrict__ pid,
-                                                              const R *__restrict__ w, const R *__restrict__ vel,
-                                                              R *__restrict__ out) {
-  const int fI = IM + (dir == 0), fJ = JM + (dir == 1), fK = KM + (dir == 2);
-  const int64_t nf = (int64_t)fI * fJ * fK;
-  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
-  int i = (int)(f % fI);
-  int64_t r = f / fI;
-  int j = (int)(r % fJ);
-  int k = (int)(r / fJ);
-  Acc<R> a0, a1, a2, a3;
-#pragma unroll
-  for (int side = 0; side < 2; ++side) {
-    int ci = i, cj = j, ck = k;
-    if (side == 0) {
-      if (dir == 0) ci -= 1;
-      if (dir == 1) cj -= 1;
-      if (dir == 2) ck -= 1;
+  if (L == 2) {
+    const int64_t p0 = idx[s], p1 = idx[s + 1];
+    if (pid[p0] < pid[p1]) {
+      f(p0);
+      f(p1);
+    } else {
+      f(p1);
+      f(p0);
     }
-    if (ci < 0 || ci >= IM || cj < 0 || cj >= JM || ck < 0 || ck >= KM) continue;
-    int64_t c = (int64_t)ci + (int64_t)IM * (cj + (int64_t)JM * ck);
-    int64_t s = start[c], e = start[c + 1];
-    if (e - s == 1) {
-      int64_t q = idx[s];
-      R av = w[3 * q + dir];
-      R wt = side == 0 ? av : (R)1 - av;
-      a0.add(wt);
-      a1.add(wt * vel[3 * q]);
-      a2.add(wt * vel[3 * q + 1]);
-      a3.add(wt * vel[3 * q + 2]);
-      continue;
-    }
-    int64_t last = INT64_MIN, lp;
-    for (int64_t t = s; t < e; ++t) {
-      int64_t q = next_by_pid_idx(idx, pid, s, e, last, lp);
-      last = lp;
-      R av = w[3 * q + dir];
-      R wt = side == 0 ? av : (R)1 - av;
-      a0.add(wt);
-      a1.add(wt * vel[3 * q]);
-      a2.add(wt * vel[3 * q + 1]);
-      a3.add(wt * vel[3 * q + 2]);
-    }
+    return;
   }
-  R *o = out + 4 * f;
-  o[0] = a0.value();
-  o[1] = a1.value();
-  o[2] = a2.value();
-  o[3] = a3.value();
-}
-
-template <typename R>
-__global__ __launch_bounds__(kBlock) void k_scatter_P_idx(int64_t ncell, const int32_t *__restrict__ start,
-                                                          const int32_t *__restrict__ idx,
-                                                          const int64_t *__restrict__ pid, const R *__restrict__ P,
-                                                          R *__restrict__ avg) {
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  int64_t s = start[c], e = start[c + 1];
-  Acc<R> acc;
-  int64_t last = INT64_MIN, lp;
+  int64_t last = INT64_MIN;
   for (int64_t t = s; t < e; ++t) {
-    int64_t q = next_by_pid_idx(idx, pid, s, e, last, lp);
-    acc.add(P[q]);
-    last = lp;
+    int64_t best = -1, bp = INT64_MAX;
+    for (int64_t q = s; q < e; ++q) {
+      const int64_t v = pid[idx[q]];
+      if (v > last && v < bp) {
+        bp = v;
+        best = idx[q];
+      }
+    }
+    last = bp;
+    f(best);
   }
-  R cnt = (R)(e - s);
+}
+
+// Face deposition (no reference counterpart).  A particle in cell (i,j,k)
+// with weights (a1,a2,a3) adds (1-a1) to i-face i and a1 to i-face i+1, and
+// likewise along j and k; faces sum count and momentum.  One thread per
+// (i,j,k) of the extended range [0..IM]x[0..JM]x[0..KM] builds the lower
+// face of (i,j,k) in each direction: the lower neighbour's particles
+// (weight a) then the cell's own (weight 1-a), each in PID order.
+// Threads are mapped to cells by 8x8x4 tiles of the extended range (xm).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_deposit_cells(int IM, int JM, int KM, TileMap tm, TileMap xm,
+                                                          const int32_t *__restrict__ kstart,
+                                                          const int32_t *__restrict__ idx,
+                                                          const int64_t *__restrict__ pid, const R *__restrict__ w,
+                                                          const R *__restrict__ vel, R *__restrict__ fi,
+                                                          R *__restrict__ fj, R *__restrict__ fk) {
+  const int b = blockIdx.x;
+  const int bi = b % xm.NTI, bj = (b / xm.NTI) % xm.NTJ, bk = b / (xm.NTI * xm.NTJ);
+  const int l = threadIdx.x;
+  const int i = bi * TX + (l % TX), j = bj * TY + ((l / TX) % TY), k = bk * TZ + (l / (TX * TY));
+  if (i > IM || j > JM || k > KM) return;
+  const bool has[3] = {j < JM && k < KM, i < IM && k < KM, i < IM && j < JM};
+  Acc<R> acc[3][4];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (!has[d]) continue;
+    const int li = i - (d == 0), lj = j - (d == 1), lk = k - (d == 2);
+    if (li < 0 || lj < 0 || lk < 0) continue;
+    const int32_t key = cell_key(tm, li, lj, lk);
+    visit_by_pid(idx, pid, kstart[key], kstart[key + 1], [&](int64_t q) {
+      const R a = w[3 * q + d];
+      acc[d][0].add(a);
+      acc[d][1].add(a * vel[3 * q]);
+      acc[d][2].add(a * vel[3 * q + 1]);
+      acc[d][3].add(a * vel[3 * q + 2]);
+    });
+  }
+  if (i < IM && j < JM && k < KM) {
+    const int32_t key = cell_key(tm, i, j, k);
+    visit_by_pid(idx, pid, kstart[key], kstart[key + 1], [&](int64_t q) {
+      const R u0 = vel[3 * q], u1 = vel[3 * q + 1], u2 = vel[3 * q + 2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const R a = (R)1 - w[3 * q + d];
+        acc[d][0].add(a);
+        acc[d][1].add(a * u0);
+        acc[d][2].add(a * u1);
+        acc[d][3].add(a * u2);
+      }
+    });
+  }
+  R *out[3] = {fi + 4 * (((int64_t)k * JM + j) * (IM + 1) + i), fj + 4 * (((int64_t)k * (JM + 1) + j) * IM + i),
+               fk + 4 * (((int64_t)k * JM + j) * IM + i)};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (!has[d]) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) out[d][v] = acc[d][v].value();
+  }
+}
+
+// P average per cell through the keyed index (PID order)
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_scatter_P_keyed(int IM, int JM, int KM, TileMap tm,
+                                                            const int32_t *__restrict__ kstart,
+                                                            const int32_t *__restrict__ idx,
+                                                            const int64_t *__restrict__ pid, const R *__restrict__ P,
+                                                            R *__restrict__ avg) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (int64_t)IM * JM * KM) return;
+  int i, j, k;
+  unpack_cell((int32_t)c, IM, JM, i, j, k);
+  const int32_t key = cell_key(tm, i, j, k);
+  const int32_t s = kstart[key], e = kstart[key + 1];
+  Acc<R> acc;
+  visit_by_pid(idx, pid, s, e, [&](int64_t q) { acc.add(P[q]); });
+  const R cnt = (R)(e - s);
   avg[c] = cnt > (R)0.5 ? acc.value() / cnt : (R)0;
 }
 
-// Physical reorder: dst[q] = src[idx[q]] for q < m (cell order, live only).
+// per-cell counts from the keyed histogram
+__global__ __launch_bounds__(kBlock) void k_count_from_keys(int IM, int JM, int KM, TileMap tm,
+                                                            const int32_t *__restrict__ kstart,
+                                                            int32_t *__restrict__ out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (int64_t)IM * JM * KM) return;
+  int i, j, k;
+  unpack_cell((int32_t)c, IM, JM, i, j, k);
+  const int32_t key = cell_key(tm, i, j, k);
+  out[c] = kstart[key + 1] - kstart[key];
+}
+
+// Physical reorder: dst[q] = src[idx[q]] for q < m.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_gather_swarm(int64_t m, const int32_t *__restrict__ idx, const double *pos,
                                                          const R *vel, const R *w, const int32_t *cell,

@@ -141,7 +141,7 @@ static int finish_grid(picgpu_ctx *c) {
   }
   c->have_grid = true;
   c->have_ucat = false;
-  c->sorted = false;
+  c->sorted = c->keyed = false;
   c->have_pavg = c->have_faces = false;
   return PICGPU_OK;
 }
@@ -222,7 +222,8 @@ int picgpu_destroy(picgpu_ctx *c) {
   DevBuf *bufs[] = {&c->coor, &c->ucat, &c->corner, &c->rank, &c->keep, &c->walk_steps, &c->guess, &c->pos0,
                     &c->vel0, &c->cell_count, &c->cell_start, &c->pavg, &c->face[0], &c->face[1], &c->face[2],
                     &c->scan_tmp, &c->counters, &c->mlp_master, &c->mlp_bf16, &c->mlp_adam_m, &c->mlp_adam_v,
-                    &c->mlp_grad, &c->mlp_partial, &c->mlp_x, &c->mlp_t, &c->mlp_y};
+                    &c->mlp_grad, &c->mlp_partial, &c->mlp_x, &c->mlp_t, &c->mlp_y, &c->kcount,
+                    &c->kstart[0], &c->kstart[1], &c->kidx[0], &c->kidx[1], &c->kslot};
   for (DevBuf *b : bufs) b->release();
   SwarmBufs *sws[] = {&c->sw, &c->alt};
   for (SwarmBufs *s : sws) {
@@ -317,7 +318,7 @@ int picgpu_init_particles_device(picgpu_ctx *c, int64_t n, const double *d_pos, 
   PG_HIP(hipSetDevice(c->device));
   PG_CHECK(ensure_swarm(c, c->sw, n));
   c->n = n;
-  c->sorted = false;
+  c->sorted = c->keyed = false;
   if (n == 0) return PICGPU_OK;
   int r = dispatch_real(c, [&](auto *tag) -> int {
     using R = std::remove_pointer_t<decltype(tag)>;
@@ -373,7 +374,7 @@ int picgpu_upload_particles(picgpu_ctx *c, int64_t n, const double *pos, const d
   PG_HIP(hipSetDevice(c->device));
   PG_CHECK(ensure_swarm(c, c->sw, n));
   c->n = n;
-  c->sorted = false;
+  c->sorted = c->keyed = false;
   if (n == 0) return PICGPU_OK;
   PG_HIP(hipMemcpy(c->sw.pos.ptr, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice));
   PG_HIP(hipMemcpy(c->sw.pid.ptr, pid, sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice));
@@ -471,7 +472,7 @@ static int do_reset_advect(picgpu_ctx *c, double dt, int reset, int advect) {
     using R = std::remove_pointer_t<decltype(tag)>;
     PG_LAUNCH(c, "reset_advect", nblocks(c->n), kBlock, k_reset_advect<R>, c->n, c->sw.pos.as<double>(),
               c->sw.vel.as<R>(), c->sw.status.as<int32_t>(), dt, reset, advect);
-    c->sorted = false;
+    c->sorted = c->keyed = false;
     return PICGPU_OK;
   });
 }
@@ -536,7 +537,7 @@ static int do_locate(picgpu_ctx *c, int skip_lost) {
   c->stats.fp64_fallbacks = (int64_t)h[C_FALLBACKS];
   c->stats.located = (int64_t)h[C_LOCATED];
   c->stats.lost = (int64_t)h[C_LOST];
-  c->sorted = false;
+  c->sorted = c->keyed = false;
   return PICGPU_OK;
 }
 
@@ -573,7 +574,7 @@ static int do_remove(picgpu_ctx *c, int check_lost, int check_box, int64_t *remo
   PG_CHECK(r);
   swap_swarm(c);
   c->n = total;
-  c->sorted = false;
+  c->sorted = c->keyed = false;
   return PICGPU_OK;
 }
 
@@ -652,6 +653,7 @@ static int do_sort(picgpu_ctx *c) {
     swap_swarm(c);
   }
   c->sorted = true;
+  c->keyed = false;
   return PICGPU_OK;
 }
 
@@ -718,49 +720,92 @@ static int do_rk2(picgpu_ctx *c, double dt) {
 }
 
 // ---------------------------------------------------------------------------
-// fused step: one pass per particle, then an index counting sort by cell and
-// deposition through the index; the swarm is compacted into cell order
-// every kCompactEvery steps and at the end of picgpu_step.
+// fused step: one pass per particle, one workgroup per 8x8x4 tile of cells,
+// particles visited through an index by tile-major cell key; each step
+// builds the next index from the slots its particles took, and deposition
+// reads that index.  The swarm is compacted into key order every
+// kCompactEvery steps and at the end of picgpu_step.
 // ---------------------------------------------------------------------------
 static constexpr int kCompactEvery = 8;
 
-static int compact_by_index(picgpu_ctx *c, int64_t live) {
+static TileMap tile_map(const picgpu_ctx *c) {
+  return TileMap{(c->IM + TX - 1) / TX, (c->JM + TY - 1) / TY, (c->KM + TZ - 1) / TZ};
+}
+static int64_t ntiles(const TileMap &t) { return (int64_t)t.NTI * t.NTJ * t.NTK; }
+
+// Index of the current swarm in kidx[kcur] / kstart[kcur]: live particles by
+// key, the live ones without a cell in the last bucket, removed ones left out.
+static int build_key_index(picgpu_ctx *c) {
+  const TileMap tm = tile_map(c);
+  const int64_t nkeys = ntiles(tm) * TS;
   const int64_t n = c->n;
-  PG_CHECK(ensure_swarm(c, c->alt, n));
+  const int cur = c->kcur;
+  const size_t m = (size_t)(n > 0 ? n : 1);
+  PG_CHECK(c->kcount.ensure(sizeof(int32_t) * (size_t)(nkeys + 2)));
+  PG_CHECK(c->kstart[cur].ensure(sizeof(int32_t) * (size_t)(nkeys + 2)));
+  PG_CHECK(c->kidx[cur].ensure(sizeof(int32_t) * m));
+  PG_CHECK(c->kslot.ensure(sizeof(int32_t) * m));
+  PG_HIP(hipMemsetAsync(c->kcount.ptr, 0, sizeof(int32_t) * (size_t)(nkeys + 1), c->stream));
+  if (n > 0)
+    PG_LAUNCH(c, "key_histogram", nblocks(n), kBlock, k_key_histogram, n, c->sw.cell.as<int32_t>(),
+              c->sw.status.as<int32_t>(), c->IM, c->JM, tm, (int32_t)nkeys, c->kcount.as<int32_t>(),
+              c->kslot.as<int32_t>());
+  PG_CHECK(scan_exclusive_i32(c, c->kcount.as<int32_t>(), c->kstart[cur].as<int32_t>(), nkeys + 1));
+  if (n > 0)
+    PG_LAUNCH(c, "key_scatter", nblocks(n), kBlock, k_key_scatter, n, c->sw.cell.as<int32_t>(),
+              c->sw.status.as<int32_t>(), c->kslot.as<int32_t>(), c->IM, c->JM, tm, (int32_t)nkeys,
+              c->kstart[cur].as<int32_t>(), c->kidx[cur].as<int32_t>());
+  int32_t h[2] = {0, 0};
+  PG_HIP(hipMemcpyAsync(h, c->kstart[cur].as<int32_t>() + nkeys, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  PG_HIP(hipStreamSynchronize(c->stream));
+  c->kbegin_unloc = h[0];
+  c->kend_unloc = h[1];
+  c->n_live = h[1];
+  c->keyed = true;
+  return PICGPU_OK;
+}
+
+// Moves the live particles into key order (the order of kidx[kcur]); the
+// index becomes the identity.
+static int compact_by_key(picgpu_ctx *c, int64_t live) {
+  PG_CHECK(ensure_swarm(c, c->alt, c->n));
+  int32_t *idx = c->kidx[c->kcur].as<int32_t>();
   if (live > 0) {
     int r = dispatch_real(c, [&](auto *tag) -> int {
       using R = std::remove_pointer_t<decltype(tag)>;
-      PG_LAUNCH(c, "gather_swarm", nblocks(live), kBlock, k_gather_swarm<R>, live, c->rank.as<int32_t>() + n,
-                c->sw.pos.as<double>(), c->sw.vel.as<R>(), c->sw.w.as<R>(), c->sw.cell.as<int32_t>(),
-                c->sw.status.as<int32_t>(), c->sw.pid.as<int64_t>(), c->sw.P.as<R>(), c->alt.pos.as<double>(),
-                c->alt.vel.as<R>(), c->alt.w.as<R>(), c->alt.cell.as<int32_t>(), c->alt.status.as<int32_t>(),
-                c->alt.pid.as<int64_t>(), c->alt.P.as<R>());
+      PG_LAUNCH(c, "gather_swarm", nblocks(live), kBlock, k_gather_swarm<R>, live, idx, c->sw.pos.as<double>(),
+                c->sw.vel.as<R>(), c->sw.w.as<R>(), c->sw.cell.as<int32_t>(), c->sw.status.as<int32_t>(),
+                c->sw.pid.as<int64_t>(), c->sw.P.as<R>(), c->alt.pos.as<double>(), c->alt.vel.as<R>(),
+                c->alt.w.as<R>(), c->alt.cell.as<int32_t>(), c->alt.status.as<int32_t>(), c->alt.pid.as<int64_t>(),
+                c->alt.P.as<R>());
       return PICGPU_OK;
     });
     PG_CHECK(r);
+    PG_LAUNCH(c, "iota", nblocks(live), kBlock, k_iota, live, idx);
   }
   swap_swarm(c);
   c->n = live;
   return PICGPU_OK;
 }
 
-static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool last) {
-  const int64_t n = c->n;
+static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, bool last) {
   const bool rk2 = (flags & PICGPU_STEP_RK2) != 0;
+  if (first || !c->keyed) PG_CHECK(build_key_index(c));
+  const TileMap tm = tile_map(c);
+  const int64_t nt = ntiles(tm), nkeys = nt * TS;
+  const int64_t n = c->n, m = c->n_live;  // m: entries of the current index
+  const int cur = c->kcur, nxt = cur ^ 1;
   PG_CHECK(counters_reset(c));
-  PG_CHECK(c->cell_count.ensure(sizeof(int32_t) * (size_t)(c->ncell + 2)));
-  PG_CHECK(c->cell_start.ensure(sizeof(int32_t) * (size_t)(c->ncell + 2)));
-  // rank buffer: [0, n) slots, [n, 2n) cell-ordered index
-  PG_CHECK(c->rank.ensure(sizeof(int32_t) * (size_t)(2 * (n > 0 ? n : 1))));
-  PG_HIP(hipMemsetAsync(c->cell_count.ptr, 0, sizeof(int32_t) * (size_t)(c->ncell + 1), c->stream));
-  int32_t *slot = c->rank.as<int32_t>();
-  int32_t *idx = slot + n;
-  if (n > 0) {
+  PG_CHECK(c->kstart[nxt].ensure(sizeof(int32_t) * (size_t)(nkeys + 2)));
+  PG_CHECK(c->kidx[nxt].ensure(sizeof(int32_t) * (size_t)(n > 0 ? n : 1)));
+  PG_HIP(hipMemsetAsync(c->kcount.ptr, 0, sizeof(int32_t) * (size_t)(nkeys + 1), c->stream));
+  const int32_t *idx = c->kidx[cur].as<int32_t>();
+  const int32_t *kstart = c->kstart[cur].as<int32_t>();
+  if (m > 0) {
     int r = dispatch_real(c, [&](auto *tag) -> int {
       using R = std::remove_pointer_t<decltype(tag)>;
       StepArgs<R> a;
       a.g = grid_view(c);
-      a.n = n;
       a.pos = c->sw.pos.as<double>();
       a.vel = c->sw.vel.as<R>();
       a.w = c->sw.w.as<R>();
@@ -777,51 +822,63 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool last) {
       a.bx1 = c->bmax[0];
       a.by1 = c->bmax[1];
       a.bz1 = c->bmax[2];
-      a.count = c->cell_count.as<int32_t>();
-      a.slot = slot;
+      a.tm = tm;
+      a.count_next = c->kcount.as<int32_t>();
+      a.slot = c->kslot.as<int32_t>();
       a.counters = c->counters.as<unsigned long long>();
       constexpr bool EX = sizeof(R) == sizeof(double);
       const char *name = EX ? "step_fused_fp64" : "step_fused_fp32";
+      const char *name_list = EX ? "step_list_fp64" : "step_list_fp32";
+      const int64_t u0 = c->kbegin_unloc, u1 = c->kend_unloc;
       if (rk2) {
-        PG_LAUNCH(c, name, nblocks(n), kBlock, (k_step_fused<R, EX, true>), a);
+        PG_LAUNCH(c, name, nt, kBlock, (k_step_tiled<R, EX, true>), a, idx, kstart);
+        if (u1 > u0) PG_LAUNCH(c, name_list, nblocks(u1 - u0), kBlock, (k_step_list<R, EX, true>), a, idx, u0, u1);
       } else {
-        PG_LAUNCH(c, name, nblocks(n), kBlock, (k_step_fused<R, EX, false>), a);
+        PG_LAUNCH(c, name, nt, kBlock, (k_step_tiled<R, EX, false>), a, idx, kstart);
+        if (u1 > u0) PG_LAUNCH(c, name_list, nblocks(u1 - u0), kBlock, (k_step_list<R, EX, false>), a, idx, u0, u1);
       }
       return PICGPU_OK;
     });
     PG_CHECK(r);
   }
-  PG_CHECK(scan_exclusive_i32(c, c->cell_count.as<int32_t>(), c->cell_start.as<int32_t>(), c->ncell));
-  if (n > 0) PG_LAUNCH(c, "index_scatter", nblocks(n), kBlock, k_index_scatter, n, c->sw.cell.as<int32_t>(), slot,
-                       c->cell_start.as<int32_t>(), idx);
+  PG_CHECK(scan_exclusive_i32(c, c->kcount.as<int32_t>(), c->kstart[nxt].as<int32_t>(), nkeys + 1));
+  if (m > 0)
+    PG_LAUNCH(c, "key_reindex", nblocks(m), kBlock, k_key_reindex, m, idx, c->sw.cell.as<int32_t>(),
+              c->kslot.as<int32_t>(), c->IM, c->JM, tm, c->kstart[nxt].as<int32_t>(), c->kidx[nxt].as<int32_t>());
   unsigned long long h[C_NUM];
   PG_CHECK(counters_read(c, h));
   if (h[C_ERROR]) {
+    c->keyed = false;
     set_error("degenerate cell face (|n| < 1e-12) met during the walk");
     return PICGPU_E_DEGENERATE;
   }
+  // every survivor of a fused step is located: the next index has no unlocated bucket
   const int64_t live = (int64_t)h[C_LOCATED];
+  c->kcur = nxt;
+  c->kbegin_unloc = c->kend_unloc = live;
+  c->n_live = live;
   c->stats.located = live;
   c->stats.removed_lost = (int64_t)h[C_REMOVED_LOST];
   c->stats.removed_out_of_bounds = (int64_t)h[C_REMOVED_OOB];
   c->stats.lost = (int64_t)h[C_REMOVED_LOST];
   c->stats.walk_iterations = (int64_t)h[C_WALK_ITERS];
   c->stats.fp64_fallbacks = (int64_t)h[C_FALLBACKS];
+  const int32_t *nidx = c->kidx[nxt].as<int32_t>();
+  const int32_t *nstart = c->kstart[nxt].as<int32_t>();
   int r = dispatch_real(c, [&](auto *tag) -> int {
     using R = std::remove_pointer_t<decltype(tag)>;
     if (flags & PICGPU_STEP_DEPOSIT_CELLS) {
       PG_CHECK(c->pavg.ensure(sizeof(R) * (size_t)c->ncell));
-      PG_LAUNCH(c, "scatter_P", nblocks(c->ncell), kBlock, k_scatter_P_idx<R>, c->ncell, c->cell_start.as<int32_t>(),
-                idx, c->sw.pid.as<int64_t>(), c->sw.P.as<R>(), c->pavg.as<R>());
+      PG_LAUNCH(c, "scatter_P", nblocks(c->ncell), kBlock, k_scatter_P_keyed<R>, c->IM, c->JM, c->KM, tm, nstart,
+                nidx, c->sw.pid.as<int64_t>(), c->sw.P.as<R>(), c->pavg.as<R>());
       c->have_pavg = true;
     }
     if (flags & PICGPU_STEP_DEPOSIT_FACES) {
-      for (int dir = 0; dir < 3; ++dir) {
-        PG_CHECK(c->face[dir].ensure(sizeof(R) * 4 * (size_t)nface(c, dir)));
-        PG_LAUNCH(c, "deposit_faces", nblocks(nface(c, dir)), kBlock, k_deposit_faces_idx<R>, dir, c->IM, c->JM,
-                  c->KM, c->cell_start.as<int32_t>(), idx, c->sw.pid.as<int64_t>(), c->sw.w.as<R>(),
-                  c->sw.vel.as<R>(), c->face[dir].as<R>());
-      }
+      for (int dir = 0; dir < 3; ++dir) PG_CHECK(c->face[dir].ensure(sizeof(R) * 4 * (size_t)nface(c, dir)));
+      const TileMap xm{c->IM / TX + 1, c->JM / TY + 1, c->KM / TZ + 1};  // tiles over [0..IM]x[0..JM]x[0..KM]
+      PG_LAUNCH(c, "deposit_faces", ntiles(xm), kBlock, k_deposit_cells<R>, c->IM, c->JM, c->KM, tm, xm, nstart, nidx,
+                c->sw.pid.as<int64_t>(), c->sw.w.as<R>(), c->sw.vel.as<R>(), c->face[0].as<R>(), c->face[1].as<R>(),
+                c->face[2].as<R>());
       c->have_faces = true;
     }
     return PICGPU_OK;
@@ -829,13 +886,10 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool last) {
   PG_CHECK(r);
   c->steps_since_compact++;
   if (last || c->steps_since_compact >= kCompactEvery) {
-    PG_CHECK(compact_by_index(c, live));
+    PG_CHECK(compact_by_key(c, live));
     c->steps_since_compact = 0;
-    // cell_start now indexes the compacted swarm directly (identity index)
-    c->sorted = true;
-  } else {
-    c->sorted = false;
   }
+  c->sorted = false;  // key order, not cell order
   return PICGPU_OK;
 }
 
@@ -927,7 +981,7 @@ int picgpu_step(picgpu_ctx *c, double dt, int nsteps, unsigned flags) {
     if (!(flags & PICGPU_STEP_STAGED)) {
       c->stats.particle_steps += (s == 0) ? c->n : c->n_live;
       PG_CHECK(do_center_to_corner(c));
-      PG_CHECK(do_step_fused(c, dt, flags, s == nsteps - 1));
+      PG_CHECK(do_step_fused(c, dt, flags, s == 0, s == nsteps - 1));
       c->n_live = c->stats.located;
       continue;
     }
@@ -980,9 +1034,16 @@ int picgpu_download_grid_field(picgpu_ctx *c, int which, double *out) {
   PG_HIP(hipStreamSynchronize(c->stream));
   const int64_t nx = c->IM + 1, ny = c->JM + 1;
   if (which == PICGPU_FIELD_PARTICLE_COUNT) {
-    if (!c->sorted) {
+    if (!c->sorted && !c->keyed) {
       set_error("ParticleCount needs picgpu_scatter/picgpu_deposit_faces/picgpu_step first");
       return PICGPU_E_STATE;
+    }
+    if (!c->sorted) {
+      // after a fused step: counts from the bucket sizes of the key index
+      PG_CHECK(c->cell_count.ensure(sizeof(int32_t) * (size_t)(c->ncell + 1)));
+      PG_LAUNCH(c, "count_from_keys", nblocks(c->ncell), kBlock, k_count_from_keys, c->IM, c->JM, c->KM,
+                tile_map(c), c->kstart[c->kcur].as<int32_t>(), c->cell_count.as<int32_t>());
+      PG_HIP(hipStreamSynchronize(c->stream));
     }
     std::vector<int32_t> h((size_t)c->ncell);
     PG_HIP(hipMemcpy(h.data(), c->cell_count.ptr, sizeof(int32_t) * (size_t)c->ncell, hipMemcpyDeviceToHost));

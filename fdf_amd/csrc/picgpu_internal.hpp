@@ -120,6 +120,16 @@ struct picgpu_ctx {
   int64_t n_live = 0;       // particles not removed (equals n outside picgpu_step)
   int steps_since_compact = 0;
 
+  // fused step: index of the live particles by tile-major cell key
+  // (double-buffered: the step reads one and builds the other)
+  picgpu::DevBuf kcount;     // int32 [nkeys+2] histogram of the next index
+  picgpu::DevBuf kstart[2];  // int32 [nkeys+2] bucket starts; bucket nkeys: live, no cell
+  picgpu::DevBuf kidx[2];    // int32 [n] particle numbers in key order
+  picgpu::DevBuf kslot;      // int32 [n] slot of each particle in its next bucket
+  int kcur = 0;              // index in use
+  bool keyed = false;        // kidx[kcur] / kstart[kcur] describe the current swarm
+  int64_t kbegin_unloc = 0, kend_unloc = 0;  // range of bucket nkeys in kidx[kcur]
+
   // per-cell / per-face outputs
   picgpu::DevBuf cell_count;  // int32 [ncell+1]
   picgpu::DevBuf cell_start;  // int32 [ncell+1]

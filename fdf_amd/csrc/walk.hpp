@@ -291,14 +291,13 @@ struct FilterResult {
   bool degenerate;
 };
 
-__device__ __forceinline__ FilterResult eval_cell_fp32(const GridView &g, int i, int j, int k, const double p[3],
-                                                       double thresh_scale, float d[6]) {
+// V: the cell's vertices (already loaded); g, (i,j,k): for the FP64 fallback.
+__device__ __forceinline__ FilterResult eval_cell_fp32_v(const double V[8][3], const GridView &g, int i, int j, int k,
+                                                         const double p[3], double thresh_scale, float d[6]) {
   FilterResult res{0, false};
   float r[8][3];
   float q[3];
   {
-    double V[8][3];
-    load_vertices(g, i, j, k, V);
 #pragma unroll
     for (int v = 0; v < 8; ++v)
 #pragma unroll
@@ -344,6 +343,13 @@ __device__ __forceinline__ FilterResult eval_cell_fp32(const GridView &g, int i,
     }
   }
   return res;
+}
+
+__device__ __forceinline__ FilterResult eval_cell_fp32(const GridView &g, int i, int j, int k, const double p[3],
+                                                       double thresh_scale, float d[6]) {
+  double V[8][3];
+  load_vertices(g, i, j, k, V);
+  return eval_cell_fp32_v(V, g, i, j, k, p, thresh_scale, d);
 }
 
 }  // namespace picgpu
