@@ -46,6 +46,7 @@ struct StepArgs {
   int32_t *cell;
   int32_t *status;
   const R *corner;
+  const void *planes;  // CellPlanes64 (FP64) or CellPlanes32 (FP32) per cell
   double dt;
   int max_traversal;
   double thresh_scale;
@@ -56,8 +57,6 @@ struct StepArgs {
   int32_t *slot;        // [n] slot in the new key bucket, -1 if removed
   unsigned long long *counters;
 };
-
-enum WalkOutcome : int { W_LOCATED = 0, W_LOST = 1, W_DEGENERATE = 2 };
 
 // corner velocities of cell (i,j,k) in the order of interpolation.c:990-999
 template <typename R>
@@ -92,12 +91,41 @@ __device__ __forceinline__ void trilinear_vals(const R C[8][3], R a1, R a2, R a3
   }
 }
 
+// Distances of point x to the faces of a cell with vertices V: the reference
+// arithmetic (EXACT) or the certified FP32 filter.  False: degenerate face.
+template <typename R, bool EXACT>
+__device__ __forceinline__ bool eval_vertices(const double V[8][3], const GridView &g, int i, int j, int k,
+                                              const double x[3], double thresh_scale, R d[6],
+                                              unsigned long long &fallbacks) {
+  if constexpr (EXACT) {
+    double dd[6];
+    if (!eval_cell_exact(V, x, thresh_scale, dd)) return false;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) d[f] = dd[f];
+  } else {
+    float dd[6];
+    FilterResult fr = eval_cell_fp32_v(V, g, i, j, k, x, thresh_scale, dd);
+    if (fr.degenerate) return false;
+    fallbacks += fr.fallbacks;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) d[f] = dd[f];
+  }
+  return true;
+}
+
 // Grid data straight from global memory.
 template <typename R>
 struct GlobalSrc {
   GridView g;
   const R *corner;
   __device__ __forceinline__ void vertices(int i, int j, int k, double V[8][3]) const { load_vertices(g, i, j, k, V); }
+  template <bool EXACT>
+  __device__ __forceinline__ bool eval(int i, int j, int k, const double x[3], double ts, R d[6],
+                                       unsigned long long &fb) const {
+    double V[8][3];
+    vertices(i, j, k, V);
+    return eval_vertices<R, EXACT>(V, g, i, j, k, x, ts, d, fb);
+  }
   __device__ __forceinline__ void corners(int i, int j, int k, R C[8][3]) const {
     const int64_t sy = g.IM + 1, sz = (int64_t)(g.JM + 1) * (g.IM + 1);
     const int64_t b = (int64_t)k * sz + (int64_t)j * sy + i;
@@ -111,14 +139,18 @@ struct GlobalSrc {
   }
 };
 
+// pointers into LDS (address space 3), so that staged reads compile to ds_read
+template <typename T>
+using lds_ptr = __attribute__((address_space(3))) const T *;
+
 // Grid data of a tile (plus halo) staged in LDS; other cells from global memory.
 template <typename R>
 struct TileSrc {
   GridView g;
   const R *corner;
-  const double *sc;  // [BZ][BY][BX][3]
-  const R *sk;       // [BZ][BY][BX][3]
-  int i0, j0, k0;    // node origin of the staged box
+  lds_ptr<double> sc;  // [BZ][BY][BX][3]
+  lds_ptr<R> sk;       // [BZ][BY][BX][3]
+  int i0, j0, k0;      // node origin of the staged box
   __device__ __forceinline__ bool staged(int i, int j, int k) const {
     return (unsigned)(i - i0) < (unsigned)(BX - 1) && (unsigned)(j - j0) < (unsigned)(BY - 1) &&
            (unsigned)(k - k0) < (unsigned)(BZ - 1);
@@ -131,7 +163,7 @@ struct TileSrc {
         const int dx = (v == 1 || v == 2 || v == 5 || v == 6) ? 1 : 0;
         const int dy = (v == 2 || v == 3 || v == 4 || v == 5) ? 1 : 0;
         const int dz = (v >= 4) ? 1 : 0;
-        const double *q = sc + 3 * (b + dx + dy * BX + dz * BX * BY);
+        lds_ptr<double> q = sc + 3 * (b + dx + dy * BX + dz * BX * BY);
         V[v][0] = q[0];
         V[v][1] = q[1];
         V[v][2] = q[2];
@@ -140,12 +172,19 @@ struct TileSrc {
       load_vertices(g, i, j, k, V);
     }
   }
+  template <bool EXACT>
+  __device__ __forceinline__ bool eval(int i, int j, int k, const double x[3], double ts, R d[6],
+                                       unsigned long long &fb) const {
+    double V[8][3];
+    vertices(i, j, k, V);
+    return eval_vertices<R, EXACT>(V, g, i, j, k, x, ts, d, fb);
+  }
   __device__ __forceinline__ void corners(int i, int j, int k, R C[8][3]) const {
     if (staged(i, j, k)) {
       const int b = ((k - k0) * BY + (j - j0)) * BX + (i - i0);
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
-        const R *q = sk + 3 * (b + (c & 1) + ((c >> 1) & 1) * BX + ((c >> 2) & 1) * BX * BY);
+        lds_ptr<R> q = sk + 3 * (b + (c & 1) + ((c >> 1) & 1) * BX + ((c >> 2) & 1) * BX * BY);
         C[c][0] = q[0];
         C[c][1] = q[1];
         C[c][2] = q[2];
@@ -156,14 +195,126 @@ struct TileSrc {
   }
 };
 
-// The walk of walkingsearch.c:1123-1212 from cell (ci,cj,ck) (negative ->
-// (0,0,0), :611-616) for one lane.  On success (ci,cj,ck) is the owning cell
-// and wv the weights.
+// Precomputed face planes of the grid (one record per cell); corner
+// velocities of a staged tile box from LDS (TILE), otherwise from global memory.
+template <typename R>
+struct PlanesOf;
+template <>
+struct PlanesOf<double> {
+  using type = CellPlanes64;
+};
+template <>
+struct PlanesOf<float> {
+  using type = CellPlanes32;
+};
+
+template <typename R, bool TILE>
+struct PlaneSrc {
+  GridView g;
+  const R *corner;
+  const typename PlanesOf<R>::type *planes;
+  lds_ptr<R> sk;   // TILE: [BZ][BY][BX][3]
+  int i0, j0, k0;  // TILE: node origin of the staged box
+  template <bool EXACT>
+  __device__ __forceinline__ bool eval(int i, int j, int k, const double x[3], double ts, R d[6],
+                                       unsigned long long &fb) const {
+    return eval_cell_planes<R, EXACT>(planes, g, i, j, k, x, ts, d, fb);
+  }
+  __device__ __forceinline__ void corners(int i, int j, int k, R C[8][3]) const {
+    if constexpr (TILE) {
+      if ((unsigned)(i - i0) < (unsigned)(BX - 1) && (unsigned)(j - j0) < (unsigned)(BY - 1) &&
+          (unsigned)(k - k0) < (unsigned)(BZ - 1)) {
+        const int b = ((k - k0) * BY + (j - j0)) * BX + (i - i0);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          lds_ptr<R> q = sk + 3 * (b + (c & 1) + ((c >> 1) & 1) * BX + ((c >> 2) & 1) * BX * BY);
+          C[c][0] = q[0];
+          C[c][1] = q[1];
+          C[c][2] = q[2];
+        }
+        return;
+      }
+    }
+    GlobalSrc<R>{g, corner}.corners(i, j, k, C);
+  }
+};
+
+// Plane records of every cell (one thread per cell).  FP64 records hold the
+// reference's own normals, face centres and zero band; FP32 records hold the
+// normals and offsets rounded once, relative to vertex 0.
+__global__ __launch_bounds__(kBlock) void k_cell_planes64(GridView g, double thresh_scale,
+                                                          CellPlanes64 *__restrict__ out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (int64_t)g.IM * g.JM * g.KM) return;
+  int i, j, k;
+  unpack_cell((int32_t)c, g.IM, g.JM, i, j, k);
+  CellPlanes64 r;
+  cell_planes_exact(g, i, j, k, thresh_scale, r);
+  out[c] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cell_planes32(GridView g, double thresh_scale,
+                                                          CellPlanes32 *__restrict__ out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (int64_t)g.IM * g.JM * g.KM) return;
+  int i, j, k;
+  unpack_cell((int32_t)c, g.IM, g.JM, i, j, k);
+  CellPlanes64 r;
+  cell_planes_exact(g, i, j, k, thresh_scale, r);
+  const double *v0 = g.coor + 3 * cell_base_node(g, i, j, k);
+  CellPlanes32 o;
+  o.v0[0] = v0[0];
+  o.v0[1] = v0[1];
+  o.v0[2] = v0[2];
+  double amax = 0.0;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const double D = r.n[f][0] * (r.fc[f][0] - v0[0]) + r.n[f][1] * (r.fc[f][1] - v0[1]) +
+                     r.n[f][2] * (r.fc[f][2] - v0[2]);
+    o.plane[f] = make_float4((float)r.n[f][0], (float)r.n[f][1], (float)r.n[f][2], (float)D);
+    amax = fmax(amax, fabs(D));
+  }
+  float thr = (float)r.thr;
+  if ((double)thr < r.thr) thr = nextafterf(thr, INFINITY);
+  o.thr = r.thr >= 0.0 ? thr : INFINITY;
+  o.amax = (float)amax * 1.0000005f;
+  out[c] = o;
+}
+
+// One iteration of the walk of walkingsearch.c:1123-1212 for one lane:
+// evaluate cell (ci,cj,ck) for x and either accept it (weights in wv), step
+// to the next cell, or give up (outside the grid, MAX_TRAVERSAL reached, or a
+// revisited cell: the reference would cycle until MAX_TRAVERSAL).
+enum WalkOutcome : int { W_LOCATED = 0, W_LOST = 1, W_DEGENERATE = 2, W_CONTINUE = 3 };
+
+template <typename R, bool EXACT, class Src>
+__device__ __forceinline__ int walk_iter(const Src &src, const double x[3], int &ci, int &cj, int &ck, int &steps,
+                                         CycleGuard &cyc, int max_traversal, double thresh_scale, R clamp, R wv[3],
+                                         unsigned long long &fallbacks) {
+  const GridView &g = src.g;
+  if (steps >= max_traversal) return W_LOST;
+  steps++;
+  if (ci < 0 || ci >= g.IM || cj < 0 || cj >= g.JM || ck < 0 || ck >= g.KM) return W_LOST;
+  if (cyc.revisit(pack_cell(ci, cj, ck, g.IM, g.JM))) {
+    steps = max_traversal;
+    return W_LOST;
+  }
+  R d[6];
+  if (!src.template eval<EXACT>(ci, cj, ck, x, thresh_scale, d, fallbacks)) return W_DEGENERATE;
+  if (point_position<R>(d) >= 0) {
+    weights_from_distances<R>(d, clamp, wv);
+    return W_LOCATED;
+  }
+  step_indices<R>(d, ci, cj, ck);
+  return W_CONTINUE;
+}
+
+// The whole walk from cell (ci,cj,ck) (negative -> (0,0,0),
+// walkingsearch.c:611-616).  On success (ci,cj,ck) is the owning cell.
 template <typename R, bool EXACT, class Src>
 __device__ __forceinline__ int walk_one(const Src &src, const double x[3], int &ci, int &cj, int &ck,
                                         int max_traversal, double thresh_scale, R clamp, R wv[3], int &steps,
                                         unsigned long long &fallbacks) {
-  const GridView &g = src.g;
   if (ci < 0 || cj < 0 || ck < 0) {
     ci = 0;
     cj = 0;
@@ -171,36 +322,11 @@ __device__ __forceinline__ int walk_one(const Src &src, const double x[3], int &
   }
   steps = 0;
   CycleGuard cyc;
-  while (steps < max_traversal) {
-    steps++;
-    if (ci < 0 || ci >= g.IM || cj < 0 || cj >= g.JM || ck < 0 || ck >= g.KM) return W_LOST;
-    if (cyc.revisit(pack_cell(ci, cj, ck, g.IM, g.JM))) {
-      steps = max_traversal;  // the reference walks the cycle until MAX_TRAVERSAL
-      return W_LOST;
-    }
-    R d[6];
-    double V[8][3];
-    src.vertices(ci, cj, ck, V);
-    if constexpr (EXACT) {
-      double dd[6];
-      if (!eval_cell_exact(V, x, thresh_scale, dd)) return W_DEGENERATE;
-#pragma unroll
-      for (int f = 0; f < 6; ++f) d[f] = dd[f];
-    } else {
-      float dd[6];
-      FilterResult fr = eval_cell_fp32_v(V, g, ci, cj, ck, x, thresh_scale, dd);
-      if (fr.degenerate) return W_DEGENERATE;
-      fallbacks += fr.fallbacks;
-#pragma unroll
-      for (int f = 0; f < 6; ++f) d[f] = dd[f];
-    }
-    if (point_position<R>(d) >= 0) {
-      weights_from_distances<R>(d, clamp, wv);
-      return W_LOCATED;
-    }
-    step_indices<R>(d, ci, cj, ck);
+  for (;;) {
+    const int r = walk_iter<R, EXACT>(src, x, ci, cj, ck, steps, cyc, max_traversal, thresh_scale, clamp, wv,
+                                      fallbacks);
+    if (r != W_CONTINUE) return r;
   }
-  return W_LOST;
 }
 
 template <typename R, class Src>
@@ -215,21 +341,40 @@ struct StepTally {
   int err = 0;
 };
 
+// A particle's state at the start of a step.
+template <typename R>
+struct PState {
+  int32_t st, c0;
+  double x0[3];
+  R u0[3];
+};
+
+template <typename R>
+__device__ __forceinline__ void load_state(const StepArgs<R> &a, int64_t p, PState<R> &s) {
+  s.st = a.status[p];
+  s.c0 = a.cell[p];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    s.x0[c] = a.pos[3 * p + c];
+    s.u0[c] = a.vel[3 * p + c];
+  }
+}
+
 // One particle's step (shared by the tiled and the generic kernel).
 template <typename R, bool EXACT, bool RK2, class Src>
-__device__ __forceinline__ void step_particle(const StepArgs<R> &a, const Src &src, int64_t p, StepTally &t) {
-  const int32_t st = a.status[p];
+__device__ __forceinline__ void step_particle(const StepArgs<R> &a, const Src &src, int64_t p, const PState<R> &ps,
+                                              StepTally &t) {
+  const int32_t st = ps.st;
   if (st == ST_REMOVED) {
     a.slot[p] = -1;
     return;
   }
-  const double x0[3] = {a.pos[3 * p], a.pos[3 * p + 1], a.pos[3 * p + 2]};
-  const R u0[3] = {a.vel[3 * p], a.vel[3 * p + 1], a.vel[3 * p + 2]};
+  const double *x0 = ps.x0;
+  const R *u0 = ps.u0;
   int ci = -1, cj = -1, ck = -1;
-  const int32_t c0 = a.cell[p];
-  if (c0 >= 0) unpack_cell(c0, a.g.IM, a.g.JM, ci, cj, ck);
+  if (ps.c0 >= 0) unpack_cell(ps.c0, a.g.IM, a.g.JM, ci, cj, ck);
   double x1[3];
-  R wv[3] = {a.w[3 * p], a.w[3 * p + 1], a.w[3 * p + 2]};
+  R wv[3];  // set by a walk that locates
   int outcome;
   int s1 = 0, s2 = 0;
   if constexpr (RK2) {
@@ -312,6 +457,7 @@ __device__ __forceinline__ void flush_tally(StepTally &t, unsigned long long *co
     v_err |= __shfl_xor(v_err, off);
   }
   if ((threadIdx.x & 63) == 0) {
+    counters = counter_stripe(counters);
     if (v_loc) atomicAdd(&counters[C_LOCATED], v_loc);
     if (v_lost) atomicAdd(&counters[C_REMOVED_LOST], v_lost);
     if (v_oob) atomicAdd(&counters[C_REMOVED_OOB], v_oob);
@@ -321,36 +467,170 @@ __device__ __forceinline__ void flush_tally(StepTally &t, unsigned long long *co
   }
 }
 
+// Slots taken in a tile's own LDS histogram carry this flag; the tile's base
+// in each bucket (kbase) is added when the next index is built.
+constexpr int32_t kLocalSlot = 0x40000000;
+
 // One workgroup per tile: stage the tile's grid data in LDS, then step the
-// tile's particles (idx[kstart[t*TS] .. kstart[(t+1)*TS])).
+// tile's particles idx[kstart[t*TS] .. kstart[(t+1)*TS]), lane l taking
+// particles l, l + kBlock, ...  Each lane runs one loop over all its walk
+// iterations (predictor walk, corrector walk, next particle), so that the
+// lanes of a wavefront evaluate cells together whatever the walk lengths of
+// their particles; the next particle's state is requested one particle ahead.
+// Particles that stay in the tile count into an LDS histogram of the tile's
+// TS keys, reserved in the global histogram once per key at the end.
 template <typename R, bool EXACT, bool RK2>
 __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int32_t *__restrict__ idx,
-                                                       const int32_t *__restrict__ kstart) {
-  __shared__ double sc[BN * 3];
+                                                       const int32_t *__restrict__ kstart,
+                                                       int32_t *__restrict__ kbase) {
+  static_assert(TS == kBlock, "one thread per key of the tile");
   __shared__ R sk[BN * 3];
+  __shared__ int32_t scount[TS];
   const int t = blockIdx.x;
   const int32_t s = kstart[(int64_t)t * TS], e = kstart[(int64_t)(t + 1) * TS];
   if (s == e) return;  // empty tile (uniform over the block)
   const int ti = t % a.tm.NTI, tj = (t / a.tm.NTI) % a.tm.NTJ, tk = t / (a.tm.NTI * a.tm.NTJ);
-  TileSrc<R> src{a.g, a.corner, sc, sk, ti * TX - 1, tj * TY - 1, tk * TZ - 1};
+  PlaneSrc<R, true> src{a.g, a.corner, (const typename PlanesOf<R>::type *)a.planes, (lds_ptr<R>)sk,
+                        ti * TX - 1, tj * TY - 1, tk * TZ - 1};
+  const int32_t B = (int32_t)blockDim.x;
+  // first particle, and the number of the second, requested before staging
+  int32_t q = s + (int32_t)threadIdx.x;
+  int32_t p = q < e ? idx[q] : -1;
+  int32_t pn = q + B < e ? idx[q + B] : -1;
+  PState<R> ps;
+  if (p >= 0) load_state(a, p, ps);
   const int64_t sy = a.g.IM + 1, sz = (int64_t)(a.g.JM + 1) * (a.g.IM + 1);
-  for (int q = threadIdx.x; q < BN; q += blockDim.x) {
-    const int bx = q % BX, by = (q / BX) % BY, bz = q / (BX * BY);
+  for (int u = threadIdx.x; u < BN; u += blockDim.x) {
+    const int bx = u % BX, by = (u / BX) % BY, bz = u / (BX * BY);
     const int ni = src.i0 + bx, nj = src.j0 + by, nk = src.k0 + bz;
     if (ni >= 0 && ni <= a.g.IM && nj >= 0 && nj <= a.g.JM && nk >= 0 && nk <= a.g.KM) {
       const int64_t node = (int64_t)nk * sz + (int64_t)nj * sy + ni;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        sc[3 * q + c] = a.g.coor[3 * node + c];
-        sk[3 * q + c] = a.corner[3 * node + c];
-      }
+      for (int c = 0; c < 3; ++c) sk[3 * u + c] = a.corner[3 * node + c];
     }
   }
+  scount[threadIdx.x] = 0;
   __syncthreads();
+
   StepTally tally;
-  for (int32_t q = s + (int32_t)threadIdx.x; q < e; q += (int32_t)blockDim.x)
-    step_particle<R, EXACT, RK2>(a, src, (int64_t)idx[q], tally);
+  const double h = 0.5 * a.dt;
+  PState<R> pnext;  // state of particle pn, requested one particle ahead
+  int32_t pnn = -1;
+  // per-particle walk state; phase 0: no particle, 1: predictor walk (RK2),
+  // 2: final walk, 3: LOST at step start (RK2; removed without a walk)
+  int ph = 0;
+  double xw[3] = {0, 0, 0};
+  int ci = 0, cj = 0, ck = 0, steps = 0, stot = 0;
+  CycleGuard cyc;
+  R wv[3] = {0, 0, 0};
+  int32_t fp = -1, fv = 0;  // deferred store of a slot taken in the global histogram
+
+  auto begin = [&]() {
+    // request the next particle's state, then set up the walk of p
+    if (pn >= 0) load_state(a, pn, pnext);
+    pnn = q + 2 * B < e ? idx[q + 2 * B] : -1;
+    ci = cj = ck = 0;
+    if (ps.c0 >= 0) unpack_cell(ps.c0, a.g.IM, a.g.JM, ci, cj, ck);
+    steps = stot = 0;
+    cyc = CycleGuard{};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xw[c] = ps.x0[c] + (double)ps.u0[c] * a.dt;
+    if (ps.st == ST_REMOVED)
+      ph = 4;
+    else if (RK2 && ps.st == ST_LOST)
+      ph = 3;
+    else
+      ph = RK2 ? 1 : 2;
+  };
+  if (p >= 0) begin();
+
+  while (ph != 0) {
+    int r;
+    R ui[3] = {0, 0, 0};
+    if (ph >= 3) {
+      r = W_LOST;
+    } else {
+      r = walk_iter<R, EXACT>(src, xw, ci, cj, ck, steps, cyc, a.max_traversal, a.thresh_scale, a.clamp, wv,
+                              tally.fb);
+      if (r == W_CONTINUE) continue;
+      if (r == W_LOCATED) interp_at<R>(src, ci, cj, ck, wv, ui);
+      if (RK2 && ph == 1 && r == W_LOCATED) {
+        // corrector position; the final walk starts from the predictor's cell
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xw[c] = ps.x0[c] + ((double)ps.u0[c] + (double)ui[c]) * h;
+        ph = 2;
+        stot += steps;
+        steps = 0;
+        cyc = CycleGuard{};
+        continue;
+      }
+    }
+    // particle p is done: outcome r at position xw (x0 when LOST at start)
+    if (ph == 4) {
+      a.slot[p] = -1;
+    } else {
+      if (ph == 3) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xw[c] = ps.x0[c];
+      }
+      tally.iters += stot + steps;
+      a.pos[3 * p] = xw[0];
+      a.pos[3 * p + 1] = xw[1];
+      a.pos[3 * p + 2] = xw[2];
+      if (r == W_DEGENERATE) {
+        tally.err = 1;
+        a.slot[p] = -1;
+      } else if (r == W_LOST) {
+        tally.lost++;
+        a.cell[p] = -1;
+        a.status[p] = ST_REMOVED;
+        a.slot[p] = -1;
+      } else {
+        const bool in = xw[0] >= a.bx0 && xw[0] <= a.bx1 && xw[1] >= a.by0 && xw[1] <= a.by1 && xw[2] >= a.bz0 &&
+                        xw[2] <= a.bz1;
+        a.cell[p] = pack_cell(ci, cj, ck, a.g.IM, a.g.JM);
+        a.w[3 * p] = wv[0];
+        a.w[3 * p + 1] = wv[1];
+        a.w[3 * p + 2] = wv[2];
+        if (in) {
+          tally.located++;
+          a.vel[3 * p] = ui[0];
+          a.vel[3 * p + 1] = ui[1];
+          a.vel[3 * p + 2] = ui[2];
+          a.status[p] = ST_LOCATED;
+          const int32_t key = cell_key(a.tm, ci, cj, ck);
+          if (key / TS == t) {
+            a.slot[p] = atomicAdd(&scount[key - t * TS], 1) | kLocalSlot;
+          } else {
+            if (fp >= 0) a.slot[fp] = fv;
+            fp = p;
+            fv = atomicAdd(&a.count_next[key], 1);
+          }
+        } else {
+          tally.oob++;
+          a.status[p] = ST_REMOVED;
+          a.slot[p] = -1;
+        }
+      }
+    }
+    // next particle
+    q += B;
+    p = pn;
+    pn = pnn;
+    ph = 0;
+    if (p >= 0) {
+      ps = pnext;
+      begin();
+    }
+  }
+  if (fp >= 0) a.slot[fp] = fv;
   flush_tally(tally, a.counters);
+  __syncthreads();
+  const int32_t cnt = scount[threadIdx.x];
+  if (cnt > 0) {
+    const int64_t key = (int64_t)t * TS + threadIdx.x;
+    kbase[key] = atomicAdd(&a.count_next[key], cnt);
+  }
 }
 
 // Particles idx[q0 .. q1) from global memory (particles without a cell).
@@ -359,7 +639,13 @@ __global__ __launch_bounds__(kBlock) void k_step_list(StepArgs<R> a, const int32
                                                       int64_t q1) {
   const int64_t q = q0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   StepTally tally;
-  if (q < q1) step_particle<R, EXACT, RK2>(a, GlobalSrc<R>{a.g, a.corner}, (int64_t)idx[q], tally);
+  if (q < q1) {
+    const int64_t p = idx[q];
+    PState<R> ps;
+    load_state(a, p, ps);
+    const PlaneSrc<R, false> src{a.g, a.corner, (const typename PlanesOf<R>::type *)a.planes, nullptr, 0, 0, 0};
+    step_particle<R, EXACT, RK2>(a, src, p, ps, tally);
+  }
   flush_tally(tally, a.counters);
 }
 
@@ -400,20 +686,24 @@ __global__ __launch_bounds__(kBlock) void k_key_scatter(int64_t n, const int32_t
 }
 
 // After a step: idx_next[kstart_next[key(cell[p])] + slot[p]] = p for the
-// particles p = idx_cur[0..m) that survived it (slot >= 0).
+// particles p = idx_cur[0..m) that survived it (slot >= 0; a slot in a
+// tile's own histogram is offset by the tile's base kbase[key]).
 __global__ __launch_bounds__(kBlock) void k_key_reindex(int64_t m, const int32_t *__restrict__ idx_cur,
                                                         const int32_t *__restrict__ cell,
-                                                        const int32_t *__restrict__ slot, int IM, int JM, TileMap tm,
+                                                        const int32_t *__restrict__ slot,
+                                                        const int32_t *__restrict__ kbase, int IM, int JM, TileMap tm,
                                                         const int32_t *__restrict__ kstart,
                                                         int32_t *__restrict__ idx_next) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= m) return;
   const int32_t p = idx_cur[q];
-  const int32_t s = slot[p];
+  int32_t s = slot[p];
   if (s < 0) return;
   int i, j, k;
   unpack_cell(cell[p], IM, JM, i, j, k);
-  idx_next[kstart[cell_key(tm, i, j, k)] + s] = p;
+  const int32_t key = cell_key(tm, i, j, k);
+  if (s & kLocalSlot) s = kbase[key] + (s & (kLocalSlot - 1));
+  idx_next[kstart[key] + s] = p;
 }
 
 __global__ __launch_bounds__(kBlock) void k_iota(int64_t n, int32_t *__restrict__ idx) {
@@ -421,41 +711,43 @@ __global__ __launch_bounds__(kBlock) void k_iota(int64_t n, int32_t *__restrict_
   if (q < n) idx[q] = (int32_t)q;
 }
 
-// Calls f(p) for the particles idx[s..e) in ascending PID order (the
-// summation order of the oracle); cells of one or two particles, the common
-// case, take a direct path.
-template <class F>
-__device__ __forceinline__ void visit_by_pid(const int32_t *__restrict__ idx, const int64_t *__restrict__ pid,
-                                             int64_t s, int64_t e, F &&f) {
-  const int64_t L = e - s;
-  if (L <= 0) return;
-  if (L == 1) {
-    f((int64_t)idx[s]);
-    return;
-  }
-  if (L == 2) {
-    const int64_t p0 = idx[s], p1 = idx[s + 1];
-    if (pid[p0] < pid[p1]) {
-      f(p0);
-      f(p1);
-    } else {
-      f(p1);
-      f(p0);
+// Puts every bucket idx[kstart[key] .. kstart[key+1]) in ascending PID order,
+// the summation order of deposition (and of the oracle).  Buckets of up to 8
+// particles are ranked in registers; longer ones, rare at a few particles per
+// cell, are insertion-sorted in place.
+__global__ __launch_bounds__(kBlock) void k_bucket_pid_order(int64_t nkeys, const int32_t *__restrict__ kstart,
+                                                             int32_t *__restrict__ idx,
+                                                             const int64_t *__restrict__ pid) {
+  const int64_t key = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (key >= nkeys) return;
+  const int32_t s = kstart[key], e = kstart[key + 1];
+  const int32_t L = e - s;
+  if (L < 2) return;
+  if (L <= 8) {
+    int32_t q[8];
+    int64_t v[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) q[t] = t < L ? idx[s + t] : 0;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = t < L ? pid[q[t]] : INT64_MAX;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      int r = 0;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) r += (v[b] < v[a] || (v[b] == v[a] && b < a)) ? 1 : 0;
+      if (a < L) idx[s + r] = q[a];
     }
     return;
   }
-  int64_t last = INT64_MIN;
-  for (int64_t t = s; t < e; ++t) {
-    int64_t best = -1, bp = INT64_MAX;
-    for (int64_t q = s; q < e; ++q) {
-      const int64_t v = pid[idx[q]];
-      if (v > last && v < bp) {
-        bp = v;
-        best = idx[q];
-      }
+  for (int32_t t = s + 1; t < e; ++t) {
+    const int32_t x = idx[t];
+    const int64_t px = pid[x];
+    int32_t u = t - 1;
+    while (u >= s && pid[idx[u]] > px) {
+      idx[u + 1] = idx[u];
+      --u;
     }
-    last = bp;
-    f(best);
+    idx[u + 1] = x;
   }
 }
 
@@ -464,13 +756,13 @@ __device__ __forceinline__ void visit_by_pid(const int32_t *__restrict__ idx, co
 // likewise along j and k; faces sum count and momentum.  One thread per
 // (i,j,k) of the extended range [0..IM]x[0..JM]x[0..KM] builds the lower
 // face of (i,j,k) in each direction: the lower neighbour's particles
-// (weight a) then the cell's own (weight 1-a), each in PID order.
+// (weight a) then the cell's own (weight 1-a), each in PID order (buckets
+// ordered by k_bucket_pid_order).
 // Threads are mapped to cells by 8x8x4 tiles of the extended range (xm).
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_deposit_cells(int IM, int JM, int KM, TileMap tm, TileMap xm,
                                                           const int32_t *__restrict__ kstart,
-                                                          const int32_t *__restrict__ idx,
-                                                          const int64_t *__restrict__ pid, const R *__restrict__ w,
+                                                          const int32_t *__restrict__ idx, const R *__restrict__ w,
                                                           const R *__restrict__ vel, R *__restrict__ fi,
                                                           R *__restrict__ fj, R *__restrict__ fk) {
   const int b = blockIdx.x;
@@ -486,17 +778,21 @@ __global__ __launch_bounds__(kBlock) void k_deposit_cells(int IM, int JM, int KM
     const int li = i - (d == 0), lj = j - (d == 1), lk = k - (d == 2);
     if (li < 0 || lj < 0 || lk < 0) continue;
     const int32_t key = cell_key(tm, li, lj, lk);
-    visit_by_pid(idx, pid, kstart[key], kstart[key + 1], [&](int64_t q) {
+    const int32_t e = kstart[key + 1];
+    for (int32_t t = kstart[key]; t < e; ++t) {
+      const int64_t q = idx[t];
       const R a = w[3 * q + d];
       acc[d][0].add(a);
       acc[d][1].add(a * vel[3 * q]);
       acc[d][2].add(a * vel[3 * q + 1]);
       acc[d][3].add(a * vel[3 * q + 2]);
-    });
+    }
   }
   if (i < IM && j < JM && k < KM) {
     const int32_t key = cell_key(tm, i, j, k);
-    visit_by_pid(idx, pid, kstart[key], kstart[key + 1], [&](int64_t q) {
+    const int32_t e = kstart[key + 1];
+    for (int32_t t = kstart[key]; t < e; ++t) {
+      const int64_t q = idx[t];
       const R u0 = vel[3 * q], u1 = vel[3 * q + 1], u2 = vel[3 * q + 2];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -506,7 +802,7 @@ __global__ __launch_bounds__(kBlock) void k_deposit_cells(int IM, int JM, int KM
         acc[d][2].add(a * u1);
         acc[d][3].add(a * u2);
       }
-    });
+    }
   }
   R *out[3] = {fi + 4 * (((int64_t)k * JM + j) * (IM + 1) + i), fj + 4 * (((int64_t)k * (JM + 1) + j) * IM + i),
                fk + 4 * (((int64_t)k * JM + j) * IM + i)};
@@ -518,12 +814,11 @@ __global__ __launch_bounds__(kBlock) void k_deposit_cells(int IM, int JM, int KM
   }
 }
 
-// P average per cell through the keyed index (PID order)
+// P average per cell through the keyed index (buckets in PID order)
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_scatter_P_keyed(int IM, int JM, int KM, TileMap tm,
                                                             const int32_t *__restrict__ kstart,
-                                                            const int32_t *__restrict__ idx,
-                                                            const int64_t *__restrict__ pid, const R *__restrict__ P,
+                                                            const int32_t *__restrict__ idx, const R *__restrict__ P,
                                                             R *__restrict__ avg) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= (int64_t)IM * JM * KM) return;
@@ -532,7 +827,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_P_keyed(int IM, int JM, int 
   const int32_t key = cell_key(tm, i, j, k);
   const int32_t s = kstart[key], e = kstart[key + 1];
   Acc<R> acc;
-  visit_by_pid(idx, pid, s, e, [&](int64_t q) { acc.add(P[q]); });
+  for (int32_t t = s; t < e; ++t) acc.add(P[idx[t]]);
   const R cnt = (R)(e - s);
   avg[c] = cnt > (R)0.5 ? acc.value() / cnt : (R)0;
 }

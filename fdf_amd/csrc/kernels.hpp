@@ -69,6 +69,7 @@ __global__ __launch_bounds__(kBlock) void k_reset_advect(int64_t n, double *__re
 template <typename R>
 struct LocateArgs {
   GridView g;
+  const void *planes;  // CellPlanes64 (FP64) or CellPlanes32 (FP32) per cell
   int64_t n;
   int64_t chunk;
   const double *pos;
@@ -145,22 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_locate(LocateArgs<R> a) {
       steps = a.max_traversal;
     } else {
       R d[6];
-      bool degenerate;
-      if constexpr (EXACT) {
-        double V[8][3];
-        load_vertices(a.g, ci, cj, ck, V);
-        double dd[6];
-        degenerate = !eval_cell_exact(V, x, a.thresh_scale, dd);
-#pragma unroll
-        for (int f = 0; f < 6; ++f) d[f] = dd[f];
-      } else {
-        float dd[6];
-        FilterResult fr = eval_cell_fp32(a.g, ci, cj, ck, x, a.thresh_scale, dd);
-        degenerate = fr.degenerate;
-        my_fb += fr.fallbacks;
-#pragma unroll
-        for (int f = 0; f < 6; ++f) d[f] = dd[f];
-      }
+      const bool degenerate = !eval_cell_planes<R, EXACT>(a.planes, a.g, ci, cj, ck, x, a.thresh_scale, d, my_fb);
       if (degenerate) {
         my_err = 1;
         done = true;
@@ -204,12 +190,13 @@ __global__ __launch_bounds__(kBlock) void k_locate(LocateArgs<R> a) {
     my_err |= __shfl_xor(my_err, off);
   }
   if (lane == 0) {
-    atomicAdd(&a.counters[C_WALK_ITERS], my_iters);
-    atomicMax(&a.counters[C_WALK_MAX], my_max);
-    atomicAdd(&a.counters[C_FALLBACKS], my_fb);
-    atomicAdd(&a.counters[C_LOCATED], my_loc);
-    atomicAdd(&a.counters[C_LOST], my_lost);
-    if (my_err) atomicOr(&a.counters[C_ERROR], 1ull);
+    unsigned long long *ctr = counter_stripe(a.counters);
+    atomicAdd(&ctr[C_WALK_ITERS], my_iters);
+    atomicMax(&ctr[C_WALK_MAX], my_max);
+    atomicAdd(&ctr[C_FALLBACKS], my_fb);
+    atomicAdd(&ctr[C_LOCATED], my_loc);
+    atomicAdd(&ctr[C_LOST], my_lost);
+    if (my_err) atomicOr(&ctr[C_ERROR], 1ull);
   }
 }
 
@@ -344,8 +331,9 @@ __global__ __launch_bounds__(kBlock) void k_keep_flags(int64_t n, const double *
   // wave totals
   unsigned long long bl = __ballot(rl != 0), bo = __ballot(ro != 0);
   if ((threadIdx.x & 63) == 0) {
-    if (bl) atomicAdd(&counters[C_REMOVED_LOST], (unsigned long long)__popcll(bl));
-    if (bo) atomicAdd(&counters[C_REMOVED_OOB], (unsigned long long)__popcll(bo));
+    unsigned long long *ctr = counter_stripe(counters);
+    if (bl) atomicAdd(&ctr[C_REMOVED_LOST], (unsigned long long)__popcll(bl));
+    if (bo) atomicAdd(&ctr[C_REMOVED_OOB], (unsigned long long)__popcll(bo));
   }
 }
 

@@ -75,15 +75,27 @@ static int dispatch_real(picgpu_ctx *c, F &&f) {
   return f((double *)nullptr);
 }
 
+static constexpr size_t kCounterBytes = sizeof(unsigned long long) * C_NUM * kCounterStripes;
+
 static int counters_reset(picgpu_ctx *c) {
-  PG_CHECK(c->counters.ensure(sizeof(unsigned long long) * C_NUM));
-  PG_HIP(hipMemsetAsync(c->counters.ptr, 0, sizeof(unsigned long long) * C_NUM, c->stream));
+  PG_CHECK(c->counters.ensure(kCounterBytes));
+  PG_HIP(hipMemsetAsync(c->counters.ptr, 0, kCounterBytes, c->stream));
   return PICGPU_OK;
 }
 
+// sums the stripes (maximum for C_WALK_MAX)
 static int counters_read(picgpu_ctx *c, unsigned long long *h) {
-  PG_HIP(hipMemcpyAsync(h, c->counters.ptr, sizeof(unsigned long long) * C_NUM, hipMemcpyDeviceToHost, c->stream));
+  unsigned long long all[C_NUM * kCounterStripes];
+  PG_HIP(hipMemcpyAsync(all, c->counters.ptr, kCounterBytes, hipMemcpyDeviceToHost, c->stream));
   PG_HIP(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < C_NUM; ++k) {
+    unsigned long long v = 0;
+    for (int s = 0; s < kCounterStripes; ++s) {
+      const unsigned long long x = all[s * C_NUM + k];
+      v = k == C_WALK_MAX ? (x > v ? x : v) : v + x;
+    }
+    h[k] = v;
+  }
   return PICGPU_OK;
 }
 
@@ -140,6 +152,7 @@ static int finish_grid(picgpu_ctx *c) {
     c->bmax[a] = c->bmax[a] + c->opt.bbox_tolerance;
   }
   c->have_grid = true;
+  c->have_planes = false;
   c->have_ucat = false;
   c->sorted = c->keyed = false;
   c->have_pavg = c->have_faces = false;
@@ -223,7 +236,7 @@ int picgpu_destroy(picgpu_ctx *c) {
                     &c->vel0, &c->cell_count, &c->cell_start, &c->pavg, &c->face[0], &c->face[1], &c->face[2],
                     &c->scan_tmp, &c->counters, &c->mlp_master, &c->mlp_bf16, &c->mlp_adam_m, &c->mlp_adam_v,
                     &c->mlp_grad, &c->mlp_partial, &c->mlp_x, &c->mlp_t, &c->mlp_y, &c->kcount,
-                    &c->kstart[0], &c->kstart[1], &c->kidx[0], &c->kidx[1], &c->kslot};
+                    &c->kstart[0], &c->kstart[1], &c->kidx[0], &c->kidx[1], &c->kslot, &c->kbase, &c->planes};
   for (DevBuf *b : bufs) b->release();
   SwarmBufs *sws[] = {&c->sw, &c->alt};
   for (SwarmBufs *s : sws) {
@@ -477,8 +490,27 @@ static int do_reset_advect(picgpu_ctx *c, double dt, int reset, int advect) {
   });
 }
 
+// Face-plane records of every cell, computed once per grid.
+static int ensure_planes(picgpu_ctx *c) {
+  if (c->have_planes) return PICGPU_OK;
+  const size_t rec = c->opt.precision == PICGPU_FP32 ? sizeof(CellPlanes32) : sizeof(CellPlanes64);
+  PG_CHECK(c->planes.ensure(rec * (size_t)(c->ncell > 0 ? c->ncell : 1)));
+  if (c->ncell > 0) {
+    if (c->opt.precision == PICGPU_FP32) {
+      PG_LAUNCH(c, "cell_planes", nblocks(c->ncell), kBlock, k_cell_planes32, grid_view(c),
+                c->opt.distance_threshold, c->planes.as<CellPlanes32>());
+    } else {
+      PG_LAUNCH(c, "cell_planes", nblocks(c->ncell), kBlock, k_cell_planes64, grid_view(c),
+                c->opt.distance_threshold, c->planes.as<CellPlanes64>());
+    }
+  }
+  c->have_planes = true;
+  return PICGPU_OK;
+}
+
 static int do_locate(picgpu_ctx *c, int skip_lost) {
   PG_CHECK(need_grid(c));
+  PG_CHECK(ensure_planes(c));
   PG_CHECK(counters_reset(c));
   const int64_t n = c->n;
   if (n > 0) {
@@ -504,6 +536,7 @@ static int do_locate(picgpu_ctx *c, int skip_lost) {
       using R = std::remove_pointer_t<decltype(tag)>;
       LocateArgs<R> a;
       a.g = grid_view(c);
+      a.planes = c->planes.ptr;
       a.n = n;
       a.chunk = chunk;
       a.pos = c->sw.pos.as<double>();
@@ -517,11 +550,8 @@ static int do_locate(picgpu_ctx *c, int skip_lost) {
       a.thresh_scale = c->opt.distance_threshold;
       a.clamp = (R)c->opt.weight_clamp;
       a.skip_lost = skip_lost;
-      if (sizeof(R) == sizeof(double)) {
-        PG_LAUNCH(c, "locate_fp64", blocks, kBlock, (k_locate<R, true>), a);
-      } else {
-        PG_LAUNCH(c, "locate_fp32", blocks, kBlock, (k_locate<R, false>), a);
-      }
+      constexpr bool EX = sizeof(R) == sizeof(double);
+      PG_LAUNCH(c, EX ? "locate_fp64" : "locate_fp32", blocks, kBlock, (k_locate<R, EX>), a);
       return PICGPU_OK;
     });
     PG_CHECK(r);
@@ -790,6 +820,7 @@ static int compact_by_key(picgpu_ctx *c, int64_t live) {
 
 static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, bool last) {
   const bool rk2 = (flags & PICGPU_STEP_RK2) != 0;
+  PG_CHECK(ensure_planes(c));
   if (first || !c->keyed) PG_CHECK(build_key_index(c));
   const TileMap tm = tile_map(c);
   const int64_t nt = ntiles(tm), nkeys = nt * TS;
@@ -801,6 +832,8 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, b
   PG_HIP(hipMemsetAsync(c->kcount.ptr, 0, sizeof(int32_t) * (size_t)(nkeys + 1), c->stream));
   const int32_t *idx = c->kidx[cur].as<int32_t>();
   const int32_t *kstart = c->kstart[cur].as<int32_t>();
+  PG_CHECK(c->kbase.ensure(sizeof(int32_t) * (size_t)(nkeys > 0 ? nkeys : 1)));
+  int32_t *kbase = c->kbase.as<int32_t>();
   if (m > 0) {
     int r = dispatch_real(c, [&](auto *tag) -> int {
       using R = std::remove_pointer_t<decltype(tag)>;
@@ -812,6 +845,7 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, b
       a.cell = c->sw.cell.as<int32_t>();
       a.status = c->sw.status.as<int32_t>();
       a.corner = c->corner.as<R>();
+      a.planes = c->planes.ptr;
       a.dt = dt;
       a.max_traversal = c->opt.max_traversal;
       a.thresh_scale = c->opt.distance_threshold;
@@ -831,10 +865,10 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, b
       const char *name_list = EX ? "step_list_fp64" : "step_list_fp32";
       const int64_t u0 = c->kbegin_unloc, u1 = c->kend_unloc;
       if (rk2) {
-        PG_LAUNCH(c, name, nt, kBlock, (k_step_tiled<R, EX, true>), a, idx, kstart);
+        PG_LAUNCH(c, name, nt, kBlock, (k_step_tiled<R, EX, true>), a, idx, kstart, kbase);
         if (u1 > u0) PG_LAUNCH(c, name_list, nblocks(u1 - u0), kBlock, (k_step_list<R, EX, true>), a, idx, u0, u1);
       } else {
-        PG_LAUNCH(c, name, nt, kBlock, (k_step_tiled<R, EX, false>), a, idx, kstart);
+        PG_LAUNCH(c, name, nt, kBlock, (k_step_tiled<R, EX, false>), a, idx, kstart, kbase);
         if (u1 > u0) PG_LAUNCH(c, name_list, nblocks(u1 - u0), kBlock, (k_step_list<R, EX, false>), a, idx, u0, u1);
       }
       return PICGPU_OK;
@@ -844,7 +878,10 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, b
   PG_CHECK(scan_exclusive_i32(c, c->kcount.as<int32_t>(), c->kstart[nxt].as<int32_t>(), nkeys + 1));
   if (m > 0)
     PG_LAUNCH(c, "key_reindex", nblocks(m), kBlock, k_key_reindex, m, idx, c->sw.cell.as<int32_t>(),
-              c->kslot.as<int32_t>(), c->IM, c->JM, tm, c->kstart[nxt].as<int32_t>(), c->kidx[nxt].as<int32_t>());
+              c->kslot.as<int32_t>(), kbase, c->IM, c->JM, tm, c->kstart[nxt].as<int32_t>(), c->kidx[nxt].as<int32_t>());
+  if (m > 0 && (flags & (PICGPU_STEP_DEPOSIT_CELLS | PICGPU_STEP_DEPOSIT_FACES)))
+    PG_LAUNCH(c, "bucket_pid_order", nblocks(nkeys), kBlock, k_bucket_pid_order, nkeys, c->kstart[nxt].as<int32_t>(),
+              c->kidx[nxt].as<int32_t>(), c->sw.pid.as<int64_t>());
   unsigned long long h[C_NUM];
   PG_CHECK(counters_read(c, h));
   if (h[C_ERROR]) {
@@ -870,15 +907,14 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, b
     if (flags & PICGPU_STEP_DEPOSIT_CELLS) {
       PG_CHECK(c->pavg.ensure(sizeof(R) * (size_t)c->ncell));
       PG_LAUNCH(c, "scatter_P", nblocks(c->ncell), kBlock, k_scatter_P_keyed<R>, c->IM, c->JM, c->KM, tm, nstart,
-                nidx, c->sw.pid.as<int64_t>(), c->sw.P.as<R>(), c->pavg.as<R>());
+                nidx, c->sw.P.as<R>(), c->pavg.as<R>());
       c->have_pavg = true;
     }
     if (flags & PICGPU_STEP_DEPOSIT_FACES) {
       for (int dir = 0; dir < 3; ++dir) PG_CHECK(c->face[dir].ensure(sizeof(R) * 4 * (size_t)nface(c, dir)));
       const TileMap xm{c->IM / TX + 1, c->JM / TY + 1, c->KM / TZ + 1};  // tiles over [0..IM]x[0..JM]x[0..KM]
       PG_LAUNCH(c, "deposit_faces", ntiles(xm), kBlock, k_deposit_cells<R>, c->IM, c->JM, c->KM, tm, xm, nstart, nidx,
-                c->sw.pid.as<int64_t>(), c->sw.w.as<R>(), c->sw.vel.as<R>(), c->face[0].as<R>(), c->face[1].as<R>(),
-                c->face[2].as<R>());
+                c->sw.w.as<R>(), c->sw.vel.as<R>(), c->face[0].as<R>(), c->face[1].as<R>(), c->face[2].as<R>());
       c->have_faces = true;
     }
     return PICGPU_OK;

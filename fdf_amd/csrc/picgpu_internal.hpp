@@ -83,6 +83,14 @@ enum Counter : int {
   C_NUM = 16
 };
 
+// Device counters are striped over kCounterStripes copies (by workgroup
+// number), so that the per-wavefront atomics of a large launch do not all
+// meet on one address; the host sums the stripes.
+constexpr int kCounterStripes = 64;
+__device__ __forceinline__ unsigned long long *counter_stripe(unsigned long long *counters) {
+  return counters + (blockIdx.x & (kCounterStripes - 1)) * C_NUM;
+}
+
 struct TimerRec {
   std::string name;
   hipEvent_t a, b;
@@ -102,6 +110,8 @@ struct picgpu_ctx {
   int64_t ncell = 0, nnode = 0;
   picgpu::DevBuf coor;  // double [nnode][3]
   double bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+  picgpu::DevBuf planes;  // [ncell] CellPlanes64 (FP64) or CellPlanes32 (FP32)
+  bool have_planes = false;
 
   // Eulerian fields
   bool have_ucat = false;
@@ -126,6 +136,7 @@ struct picgpu_ctx {
   picgpu::DevBuf kstart[2];  // int32 [nkeys+2] bucket starts; bucket nkeys: live, no cell
   picgpu::DevBuf kidx[2];    // int32 [n] particle numbers in key order
   picgpu::DevBuf kslot;      // int32 [n] slot of each particle in its next bucket
+  picgpu::DevBuf kbase;      // int32 [nkeys] a tile's base in each of its own buckets
   int kcur = 0;              // index in use
   bool keyed = false;        // kidx[kcur] / kstart[kcur] describe the current swarm
   int64_t kbegin_unloc = 0, kend_unloc = 0;  // range of bucket nkeys in kidx[kcur]
@@ -142,7 +153,7 @@ struct picgpu_ctx {
   picgpu::DevBuf scan_tmp;
 
   // counters
-  picgpu::DevBuf counters;  // uint64 [C_NUM]
+  picgpu::DevBuf counters;  // uint64 [kCounterStripes][C_NUM]
   picgpu_stats stats{};
 
   // timing

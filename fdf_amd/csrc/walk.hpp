@@ -102,10 +102,11 @@ __device__ __forceinline__ void centroid_exact(const double V[8][3], double cc[3
   cc[2] /= 8.0;
 }
 
-// walkingsearch.c:102-227.  Returns false on a degenerate face.
-__device__ __forceinline__ bool face_distance_exact(const double *v1, const double *v2, const double *v3,
-                                                    const double *v4, const double cc[3], const double p[3],
-                                                    double thr, double &d) {
+// walkingsearch.c:102-227, the part that depends on the cell only: the unit
+// normal n (oriented away from the centroid) and the face centre fc.  Returns
+// false on a degenerate face.
+__device__ __forceinline__ bool face_plane_exact(const double *v1, const double *v2, const double *v3,
+                                                 const double *v4, const double cc[3], double n[3], double fc[3]) {
   double e1x = v2[0] - v1[0], e1y = v2[1] - v1[1], e1z = v2[2] - v1[2];
   double e2x = v4[0] - v1[0], e2y = v4[1] - v1[1], e2z = v4[2] - v1[2];
   double nx = e1y * e2z - e1z * e2y;
@@ -123,13 +124,30 @@ __device__ __forceinline__ bool face_distance_exact(const double *v1, const doub
     ny = -ny;
     nz = -nz;
   }
-  nx /= m;
-  ny /= m;
-  nz /= m;
-  double wx = fcx - p[0], wy = fcy - p[1], wz = fcz - p[2];
-  double dd = wx * nx + wy * ny + wz * nz;
+  n[0] = nx / m;
+  n[1] = ny / m;
+  n[2] = nz / m;
+  fc[0] = fcx;
+  fc[1] = fcy;
+  fc[2] = fcz;
+  return true;
+}
+
+// walkingsearch.c:102-227, the part that depends on the point.
+__device__ __forceinline__ double face_point_distance(const double n[3], const double fc[3], const double p[3],
+                                                      double thr) {
+  double wx = fc[0] - p[0], wy = fc[1] - p[1], wz = fc[2] - p[2];
+  double dd = wx * n[0] + wy * n[1] + wz * n[2];
   if (fabs(dd) < thr) dd = 0.0;
-  d = dd;
+  return dd;
+}
+
+__device__ __forceinline__ bool face_distance_exact(const double *v1, const double *v2, const double *v3,
+                                                    const double *v4, const double cc[3], const double p[3],
+                                                    double thr, double &d) {
+  double n[3], fc[3];
+  if (!face_plane_exact(v1, v2, v3, v4, cc, n, fc)) return false;
+  d = face_point_distance(n, fc, p, thr);
   return true;
 }
 
@@ -200,13 +218,14 @@ __device__ __forceinline__ void weights_from_distances(R d[6], R clamp, R w[3]) 
 // With u = 2^-24, Lc = max |component| of the vertex coordinates relative to
 // V0 and Lw = max(Lc, max |component| of p - V0), a first-order rounding
 // analysis of the FP32 evaluation (inputs rounded once to FP32, every
-// product, sum, sqrt and quotient rounded once) bounds
-//   |d32 - d|      <= u * Lw * (846 * Lc^2 / m + 7)
+// product and sum rounded once, sqrt and reciprocal within 1 ulp as
+// v_sqrt_f32 / v_rcp_f32 guarantee) bounds
+//   |d32 - d|      <= u * Lw * (846 * Lc^2 / m + 21)
 //   |dot32 - dot|  <= u * 600 * Lc^3
-//   |m32 - m|      <= u * (84 * Lc^2 + m)
+//   |m32 - m|      <= u * (84 * Lc^2 + 2 m)
 // where d, dot, m are the exact values on the FP64 inputs; the FP64
 // reference values differ from those by ~2^-29 of the same bounds.  The
-// constants below (1024, 16, 1024, 128) keep a margin over the analysis.
+// constants below (1040, 32, 1024, 128, 3) keep a margin over the analysis.
 // Every edge has components <= 2 Lc, so the reference's zero band
 // thr = 1e-11 * h is below 1e-11 * 6 Lc.  A face is certified when the
 // decision the reference makes on it lies outside its bound.
@@ -222,17 +241,19 @@ __device__ __forceinline__ bool face_fp32(const float r[8][3], const float cc[3]
   float nx = __builtin_fmaf(e1y, e2z, -e1z * e2y);
   float ny = __builtin_fmaf(e1z, e2x, -e1x * e2z);
   float nz = __builtin_fmaf(e1x, e2y, -e1y * e2x);
-  float m = sqrtf(__builtin_fmaf(nx, nx, __builtin_fmaf(ny, ny, nz * nz)));
+  // hardware sqrt and reciprocal (1 ulp each); the bounds below carry them
+  float m = __builtin_amdgcn_sqrtf(__builtin_fmaf(nx, nx, __builtin_fmaf(ny, ny, nz * nz)));
+  const float rm = __builtin_amdgcn_rcpf(m);
   float fcx = 0.25f * ((v1[0] + v2[0]) + (v3[0] + v4[0]));
   float fcy = 0.25f * ((v1[1] + v2[1]) + (v3[1] + v4[1]));
   float fcz = 0.25f * ((v1[2] + v2[2]) + (v3[2] + v4[2]));
   float dot = __builtin_fmaf(nx, cc[0] - fcx, __builtin_fmaf(ny, cc[1] - fcy, nz * (cc[2] - fcz)));
   float dn = __builtin_fmaf(nx, fcx - q[0], __builtin_fmaf(ny, fcy - q[1], nz * (fcz - q[2])));
-  const float Em = u * (128.f * Lc2 + 2.f * m);
+  const float Em = u * (128.f * Lc2 + 3.f * m);
   const float Edot = u * 1024.f * Lc3;
   const float flip = (dot > 1.0e-9f) ? -1.f : 1.f;
-  const float dd = flip * (dn / m);
-  const float Ed = u * Lw * (1024.f * Lc2 / m + 16.f);
+  const float dd = flip * (dn * rm);
+  const float Ed = u * Lw * (1040.f * Lc2 * rm + 32.f);
   d[D::slot] = dd;
   return (m - Em > 1.0e-12f * 1.001f) && (fabsf(dot - 1.0e-9f) > Edot + 1.0e-15f) && (fabsf(dd) > thrU + Ed);
 }
@@ -291,6 +312,28 @@ struct FilterResult {
   bool degenerate;
 };
 
+// Recomputes the faces flagged in need (bit f: face f in evaluation order) of
+// cell (i,j,k) in FP64 with the reference arithmetic.
+__device__ __forceinline__ void exact_fallback(const GridView &g, int i, int j, int k, const double p[3],
+                                               double thresh_scale, unsigned need, float d[6], FilterResult &res) {
+  const ExactCell e = exact_cell_info(g, i, j, k, thresh_scale);
+  const int64_t base = cell_base_node(g, i, j, k);
+#pragma unroll 1
+  for (int f = 0; f < 6; ++f) {
+    if (!((need >> f) & 1u)) continue;
+    const ExactFace ef = exact_face(g.coor, g.IM, g.JM, base, f, p[0], p[1], p[2], e.thr, e.c0, e.c1, e.c2);
+    if (!ef.ok) {
+      res.degenerate = true;
+      return;
+    }
+    float df = (float)ef.d;
+    // keep the exact classification: a nonzero FP64 distance never maps to 0
+    if (ef.d != 0.0 && df == 0.f) df = ef.d < 0.0 ? -1e-30f : 1e-30f;
+    set_face_slot(d, f, df);
+    res.fallbacks++;
+  }
+}
+
 // V: the cell's vertices (already loaded); g, (i,j,k): for the FP64 fallback.
 __device__ __forceinline__ FilterResult eval_cell_fp32_v(const double V[8][3], const GridView &g, int i, int j, int k,
                                                          const double p[3], double thresh_scale, float d[6]) {
@@ -324,24 +367,7 @@ __device__ __forceinline__ FilterResult eval_cell_fp32_v(const double V[8][3], c
   need |= face_fp32<3>(r, cc, q, Lc2, Lc3, Lw, thrU, d) ? 0u : 8u;
   need |= face_fp32<4>(r, cc, q, Lc2, Lc3, Lw, thrU, d) ? 0u : 16u;
   need |= face_fp32<5>(r, cc, q, Lc2, Lc3, Lw, thrU, d) ? 0u : 32u;
-  if (need) {
-    const ExactCell e = exact_cell_info(g, i, j, k, thresh_scale);
-    const int64_t base = cell_base_node(g, i, j, k);
-#pragma unroll 1
-    for (int f = 0; f < 6; ++f) {
-      if (!((need >> f) & 1u)) continue;
-      const ExactFace ef = exact_face(g.coor, g.IM, g.JM, base, f, p[0], p[1], p[2], e.thr, e.c0, e.c1, e.c2);
-      if (!ef.ok) {
-        res.degenerate = true;
-        return res;
-      }
-      float df = (float)ef.d;
-      // keep the exact classification: a nonzero FP64 distance never maps to 0
-      if (ef.d != 0.0 && df == 0.f) df = ef.d < 0.0 ? -1e-30f : 1e-30f;
-      set_face_slot(d, f, df);
-      res.fallbacks++;
-    }
-  }
+  if (need) exact_fallback(g, i, j, k, p, thresh_scale, need, d, res);
   return res;
 }
 
@@ -350,6 +376,126 @@ __device__ __forceinline__ FilterResult eval_cell_fp32(const GridView &g, int i,
   double V[8][3];
   load_vertices(g, i, j, k, V);
   return eval_cell_fp32_v(V, g, i, j, k, p, thresh_scale, d);
+}
+
+// ---------------------------------------------------------------------------
+// Face planes per cell.  The grid does not move during the particle steps, so
+// the cell-only part of every walk evaluation (walkingsearch.c:25-59, 102-227,
+// 287-295: unit normals, face centres, zero band) is computed once per grid,
+// with the reference arithmetic, and stored per cell in the faces' evaluation
+// order.  A walk evaluation is then six plane distances.
+// ---------------------------------------------------------------------------
+
+// FP64 record: the reference's own intermediate values, so that
+// face_point_distance on them reproduces the reference distance bit for bit.
+struct alignas(16) CellPlanes64 {
+  double n[6][3];   // unit normal, evaluation order BACK FRONT BOTTOM TOP LEFT RIGHT
+  double fc[6][3];  // face centre
+  double thr;       // zero band thresh_scale * h; negative: the cell has a degenerate face
+  double pad;
+};
+static_assert(sizeof(CellPlanes64) == 304, "CellPlanes64 layout");
+
+// FP32 record: normal and offset relative to vertex 0 of the cell,
+// D = n . (fc - V0) (FP64, rounded once), for the certified FP32 filter.
+struct alignas(16) CellPlanes32 {
+  float4 plane[6];  // (nx, ny, nz, D), evaluation order
+  double v0[3];     // vertex 0 of the cell
+  float thr;        // zero band rounded up; +inf: recompute every face in FP64
+  float amax;       // max |D| over the faces
+};
+static_assert(sizeof(CellPlanes32) == 128, "CellPlanes32 layout");
+
+template <int F>
+__device__ __forceinline__ bool plane_exact(const double V[8][3], const double cc[3], CellPlanes64 &r) {
+  using D = FaceDef<F>;
+  return face_plane_exact(V[D::a], V[D::b], V[D::c], V[D::d], cc, r.n[F], r.fc[F]);
+}
+
+// walkingsearch.c:811-859 for cell (i,j,k) without the point
+__device__ __forceinline__ void cell_planes_exact(const GridView &g, int i, int j, int k, double thresh_scale,
+                                                  CellPlanes64 &r) {
+  double V[8][3];
+  load_vertices(g, i, j, k, V);
+  const double thr = thresh_scale * char_size_exact(V);
+  double cc[3];
+  centroid_exact(V, cc);
+  const bool ok = plane_exact<0>(V, cc, r) & plane_exact<1>(V, cc, r) & plane_exact<2>(V, cc, r) &
+                  plane_exact<3>(V, cc, r) & plane_exact<4>(V, cc, r) & plane_exact<5>(V, cc, r);
+  r.thr = ok ? thr : -1.0;
+  r.pad = 0.0;
+}
+
+// FP64 evaluation on a record: the reference distances (walkingsearch.c:271-389).
+template <int F>
+__device__ __forceinline__ void plane_distance(const CellPlanes64 &r, const double p[3], double d[6]) {
+  d[FaceDef<F>::slot] = face_point_distance(r.n[F], r.fc[F], p, r.thr);
+}
+__device__ __forceinline__ bool eval_planes_exact(const CellPlanes64 &r, const double p[3], double d[6]) {
+  if (!(r.thr >= 0.0)) return false;
+  plane_distance<0>(r, p, d);
+  plane_distance<1>(r, p, d);
+  plane_distance<2>(r, p, d);
+  plane_distance<3>(r, p, d);
+  plane_distance<4>(r, p, d);
+  plane_distance<5>(r, p, d);
+  return true;
+}
+
+// FP32 filter on a record.  With q = fl32(p - V0) (FP64 difference rounded
+// once), Q = max |q_c|, u = 2^-24 and A = max |D|, the evaluation
+//   d32 = D - fma(nx, qx, fma(ny, qy, nz * qz))
+// differs from d* = (fc - p) . n (exact arithmetic on the reference's FP64 n
+// and fc) by at most u * (2 A + 10.5 Q) (rounding of D, of n, of q, two
+// roundings in the fma chain, one in the product and one in the difference);
+// the reference's FP64 evaluation differs from d* by ~2^-29 of that.  A face
+// is certified when |d32| > thr + u * (4 A + 16 Q): the reference's distance
+// then has the sign of d32 and lies outside its zero band.  Other faces, and
+// every face of a cell whose record holds thr = +inf (degenerate or extreme
+// cells), are recomputed by the out-of-line FP64 routine.
+template <int F>
+__device__ __forceinline__ bool plane_fp32(const CellPlanes32 &r, const float q[3], float band, float d[6]) {
+  const float4 pl = r.plane[F];
+  const float dd = pl.w - __builtin_fmaf(pl.x, q[0], __builtin_fmaf(pl.y, q[1], pl.z * q[2]));
+  d[FaceDef<F>::slot] = dd;
+  return fabsf(dd) > band;
+}
+
+__device__ __forceinline__ FilterResult eval_planes_fp32(const CellPlanes32 &r, const GridView &g, int i, int j, int k,
+                                                         const double p[3], double thresh_scale, float d[6]) {
+  FilterResult res{0, false};
+  const float q[3] = {(float)(p[0] - r.v0[0]), (float)(p[1] - r.v0[1]), (float)(p[2] - r.v0[2])};
+  const float Q = fmaxf(fabsf(q[0]), fmaxf(fabsf(q[1]), fabsf(q[2])));
+  const float u = 5.9604645e-08f;  // 2^-24
+  const float band = r.thr + u * (4.f * r.amax + 16.f * Q);
+  unsigned need = 0;
+  need |= plane_fp32<0>(r, q, band, d) ? 0u : 1u;
+  need |= plane_fp32<1>(r, q, band, d) ? 0u : 2u;
+  need |= plane_fp32<2>(r, q, band, d) ? 0u : 4u;
+  need |= plane_fp32<3>(r, q, band, d) ? 0u : 8u;
+  need |= plane_fp32<4>(r, q, band, d) ? 0u : 16u;
+  need |= plane_fp32<5>(r, q, band, d) ? 0u : 32u;
+  if (need) exact_fallback(g, i, j, k, p, thresh_scale, need, d, res);
+  return res;
+}
+
+// Distances of x to the faces of cell (i,j,k) from its plane record (planes
+// points to CellPlanes64 records for FP64, CellPlanes32 for FP32).  False:
+// the cell has a degenerate face.
+template <typename R, bool EXACT>
+__device__ __forceinline__ bool eval_cell_planes(const void *planes, const GridView &g, int i, int j, int k,
+                                                 const double x[3], double thresh_scale, R d[6],
+                                                 unsigned long long &fallbacks) {
+  const int64_t c = (int64_t)i + (int64_t)g.IM * ((int64_t)j + (int64_t)g.JM * k);
+  if constexpr (EXACT) {
+    static_assert(sizeof(R) == sizeof(double), "exact evaluation is FP64");
+    return eval_planes_exact(static_cast<const CellPlanes64 *>(planes)[c], x, d);
+  } else {
+    static_assert(sizeof(R) == sizeof(float), "filtered evaluation is FP32");
+    FilterResult fr = eval_planes_fp32(static_cast<const CellPlanes32 *>(planes)[c], g, i, j, k, x, thresh_scale, d);
+    fallbacks += fr.fallbacks;
+    return !fr.degenerate;
+  }
 }
 
 }  // namespace picgpu
