@@ -31,6 +31,18 @@ struct TileMap {
   int NTI, NTJ, NTK;  // tiles per axis
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
+// XCD b % 8), each with its own L2.  Tile of workgroup b: XCD x takes the x-th
+// contiguous eighth of the tiles, in order, so that the particles a
+// deposition workgroup reads from its lower neighbour tiles were read into
+// the same L2 by the workgroups just before it.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int b, int nt) {
+  const int q = nt / kXcds, r = nt % kXcds;
+  const int x = b % kXcds, k = b / kXcds;
+  return x * q + (x < r ? x : r) + k;
+}
+
 __device__ __forceinline__ int32_t cell_key(const TileMap &t, int i, int j, int k) {
   const int ti = i / TX, tj = j / TY, tk = k / TZ;
   const int li = i - ti * TX, lj = j - tj * TY, lk = k - tk * TZ;
@@ -56,6 +68,8 @@ struct StepArgs {
   int32_t *count_next;  // [nkeys] histogram of the new keys (zeroed by the caller)
   int32_t *slot;        // [n] slot in the new key bucket, -1 if removed
   unsigned long long *counters;
+  int32_t *defer_list;   // [n] particles left to k_step_deferred
+  int32_t *defer_count;  // entries of defer_list (zeroed by the caller)
 };
 
 // corner velocities of cell (i,j,k) in the order of interpolation.c:990-999
@@ -91,112 +105,32 @@ __device__ __forceinline__ void trilinear_vals(const R C[8][3], R a1, R a2, R a3
   }
 }
 
-// Distances of point x to the faces of a cell with vertices V: the reference
-// arithmetic (EXACT) or the certified FP32 filter.  False: degenerate face.
-template <typename R, bool EXACT>
-__device__ __forceinline__ bool eval_vertices(const double V[8][3], const GridView &g, int i, int j, int k,
-                                              const double x[3], double thresh_scale, R d[6],
-                                              unsigned long long &fallbacks) {
-  if constexpr (EXACT) {
-    double dd[6];
-    if (!eval_cell_exact(V, x, thresh_scale, dd)) return false;
-#pragma unroll
-    for (int f = 0; f < 6; ++f) d[f] = dd[f];
-  } else {
-    float dd[6];
-    FilterResult fr = eval_cell_fp32_v(V, g, i, j, k, x, thresh_scale, dd);
-    if (fr.degenerate) return false;
-    fallbacks += fr.fallbacks;
-#pragma unroll
-    for (int f = 0; f < 6; ++f) d[f] = dd[f];
-  }
-  return true;
-}
-
-// Grid data straight from global memory.
+// Corner velocities of cell (i,j,k) from global memory (order of
+// interpolation.c:990-999).
 template <typename R>
-struct GlobalSrc {
-  GridView g;
-  const R *corner;
-  __device__ __forceinline__ void vertices(int i, int j, int k, double V[8][3]) const { load_vertices(g, i, j, k, V); }
-  template <bool EXACT>
-  __device__ __forceinline__ bool eval(int i, int j, int k, const double x[3], double ts, R d[6],
-                                       unsigned long long &fb) const {
-    double V[8][3];
-    vertices(i, j, k, V);
-    return eval_vertices<R, EXACT>(V, g, i, j, k, x, ts, d, fb);
-  }
-  __device__ __forceinline__ void corners(int i, int j, int k, R C[8][3]) const {
-    const int64_t sy = g.IM + 1, sz = (int64_t)(g.JM + 1) * (g.IM + 1);
-    const int64_t b = (int64_t)k * sz + (int64_t)j * sy + i;
+__device__ __forceinline__ void load_corners(const GridView &g, const R *__restrict__ corner, int i, int j, int k,
+                                             R C[8][3]) {
+  const int64_t sy = g.IM + 1, sz = (int64_t)(g.JM + 1) * (g.IM + 1);
+  const int64_t b = (int64_t)k * sz + (int64_t)j * sy + i;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int64_t q = b + (c & 1) + ((c >> 1) & 1) * sy + ((c >> 2) & 1) * sz;
-      C[c][0] = corner[3 * q];
-      C[c][1] = corner[3 * q + 1];
-      C[c][2] = corner[3 * q + 2];
-    }
+  for (int c = 0; c < 8; ++c) {
+    const int64_t q = b + (c & 1) + ((c >> 1) & 1) * sy + ((c >> 2) & 1) * sz;
+    C[c][0] = corner[3 * q];
+    C[c][1] = corner[3 * q + 1];
+    C[c][2] = corner[3 * q + 2];
   }
-};
+}
 
 // pointers into LDS (address space 3), so that staged reads compile to ds_read
 template <typename T>
 using lds_ptr = __attribute__((address_space(3))) const T *;
 
-// Grid data of a tile (plus halo) staged in LDS; other cells from global memory.
-template <typename R>
-struct TileSrc {
-  GridView g;
-  const R *corner;
-  lds_ptr<double> sc;  // [BZ][BY][BX][3]
-  lds_ptr<R> sk;       // [BZ][BY][BX][3]
-  int i0, j0, k0;      // node origin of the staged box
-  __device__ __forceinline__ bool staged(int i, int j, int k) const {
-    return (unsigned)(i - i0) < (unsigned)(BX - 1) && (unsigned)(j - j0) < (unsigned)(BY - 1) &&
-           (unsigned)(k - k0) < (unsigned)(BZ - 1);
-  }
-  __device__ __forceinline__ void vertices(int i, int j, int k, double V[8][3]) const {
-    if (staged(i, j, k)) {
-      const int b = ((k - k0) * BY + (j - j0)) * BX + (i - i0);
-#pragma unroll
-      for (int v = 0; v < 8; ++v) {
-        const int dx = (v == 1 || v == 2 || v == 5 || v == 6) ? 1 : 0;
-        const int dy = (v == 2 || v == 3 || v == 4 || v == 5) ? 1 : 0;
-        const int dz = (v >= 4) ? 1 : 0;
-        lds_ptr<double> q = sc + 3 * (b + dx + dy * BX + dz * BX * BY);
-        V[v][0] = q[0];
-        V[v][1] = q[1];
-        V[v][2] = q[2];
-      }
-    } else {
-      load_vertices(g, i, j, k, V);
-    }
-  }
-  template <bool EXACT>
-  __device__ __forceinline__ bool eval(int i, int j, int k, const double x[3], double ts, R d[6],
-                                       unsigned long long &fb) const {
-    double V[8][3];
-    vertices(i, j, k, V);
-    return eval_vertices<R, EXACT>(V, g, i, j, k, x, ts, d, fb);
-  }
-  __device__ __forceinline__ void corners(int i, int j, int k, R C[8][3]) const {
-    if (staged(i, j, k)) {
-      const int b = ((k - k0) * BY + (j - j0)) * BX + (i - i0);
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        lds_ptr<R> q = sk + 3 * (b + (c & 1) + ((c >> 1) & 1) * BX + ((c >> 2) & 1) * BX * BY);
-        C[c][0] = q[0];
-        C[c][1] = q[1];
-        C[c][2] = q[2];
-      }
-    } else {
-      GlobalSrc<R>{g, corner}.corners(i, j, k, C);
-    }
-  }
-};
-
 // Precomputed face planes of the grid (one record per cell); corner
-// velocities of a staged tile box from LDS (TILE), otherwise from global memory.
+// velocities of a staged tile box from LDS (TILE), otherwise from global
+// memory.  With DEFER, a cell evaluation that the FP32 filter does not certify
+// is not recomputed in FP64 here: the walk reports EV_DEFER and the particle
+// is stepped again by a kernel that carries the FP64 routine, which keeps
+// that routine's registers out of the hot kernel.
 template <typename R>
 struct PlanesOf;
 template <>
@@ -208,7 +142,9 @@ struct PlanesOf<float> {
   using type = CellPlanes32;
 };
 
-template <typename R, bool TILE>
+enum EvalOutcome : int { EV_OK = 0, EV_DEGENERATE = 1, EV_DEFER = 2 };
+
+template <typename R, bool TILE, bool DEFER = false>
 struct PlaneSrc {
   GridView g;
   const R *corner;
@@ -216,9 +152,14 @@ struct PlaneSrc {
   lds_ptr<R> sk;   // TILE: [BZ][BY][BX][3]
   int i0, j0, k0;  // TILE: node origin of the staged box
   template <bool EXACT>
-  __device__ __forceinline__ bool eval(int i, int j, int k, const double x[3], double ts, R d[6],
-                                       unsigned long long &fb) const {
-    return eval_cell_planes<R, EXACT>(planes, g, i, j, k, x, ts, d, fb);
+  __device__ __forceinline__ int eval(int i, int j, int k, const double x[3], double ts, R d[6],
+                                      unsigned long long &fb) const {
+    if constexpr (DEFER && !EXACT) {
+      const int64_t c = (int64_t)i + (int64_t)g.IM * ((int64_t)j + (int64_t)g.JM * k);
+      return planes_fp32_distances(planes[c], x, d) ? EV_DEFER : EV_OK;
+    } else {
+      return eval_cell_planes<R, EXACT>(planes, g, i, j, k, x, ts, d, fb) ? EV_OK : EV_DEGENERATE;
+    }
   }
   __device__ __forceinline__ void corners(int i, int j, int k, R C[8][3]) const {
     if constexpr (TILE) {
@@ -235,7 +176,7 @@ struct PlaneSrc {
         return;
       }
     }
-    GlobalSrc<R>{g, corner}.corners(i, j, k, C);
+    load_corners<R>(g, corner, i, j, k, C);
   }
 };
 
@@ -285,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_planes32(GridView g, double thr
 // evaluate cell (ci,cj,ck) for x and either accept it (weights in wv), step
 // to the next cell, or give up (outside the grid, MAX_TRAVERSAL reached, or a
 // revisited cell: the reference would cycle until MAX_TRAVERSAL).
-enum WalkOutcome : int { W_LOCATED = 0, W_LOST = 1, W_DEGENERATE = 2, W_CONTINUE = 3 };
+enum WalkOutcome : int { W_LOCATED = 0, W_LOST = 1, W_DEGENERATE = 2, W_CONTINUE = 3, W_DEFER = 4 };
 
 template <typename R, bool EXACT, class Src>
 __device__ __forceinline__ int walk_iter(const Src &src, const double x[3], int &ci, int &cj, int &ck, int &steps,
@@ -300,7 +241,9 @@ __device__ __forceinline__ int walk_iter(const Src &src, const double x[3], int 
     return W_LOST;
   }
   R d[6];
-  if (!src.template eval<EXACT>(ci, cj, ck, x, thresh_scale, d, fallbacks)) return W_DEGENERATE;
+  const int ev = src.template eval<EXACT>(ci, cj, ck, x, thresh_scale, d, fallbacks);
+  if (ev == EV_DEGENERATE) return W_DEGENERATE;
+  if (ev == EV_DEFER) return W_DEFER;
   if (point_position<R>(d) >= 0) {
     weights_from_distances<R>(d, clamp, wv);
     return W_LOCATED;
@@ -490,8 +433,8 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
   const int32_t s = kstart[(int64_t)t * TS], e = kstart[(int64_t)(t + 1) * TS];
   if (s == e) return;  // empty tile (uniform over the block)
   const int ti = t % a.tm.NTI, tj = (t / a.tm.NTI) % a.tm.NTJ, tk = t / (a.tm.NTI * a.tm.NTJ);
-  PlaneSrc<R, true> src{a.g, a.corner, (const typename PlanesOf<R>::type *)a.planes, (lds_ptr<R>)sk,
-                        ti * TX - 1, tj * TY - 1, tk * TZ - 1};
+  PlaneSrc<R, true, !EXACT> src{a.g, a.corner, (const typename PlanesOf<R>::type *)a.planes, (lds_ptr<R>)sk,
+                               ti * TX - 1, tj * TY - 1, tk * TZ - 1};
   const int32_t B = (int32_t)blockDim.x;
   // first particle, and the number of the second, requested before staging
   int32_t q = s + (int32_t)threadIdx.x;
@@ -566,7 +509,10 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
       }
     }
     // particle p is done: outcome r at position xw (x0 when LOST at start)
-    if (ph == 4) {
+    if (r == W_DEFER) {
+      // stepped again from its start state by k_step_deferred
+      a.defer_list[atomicAdd(a.defer_count, 1)] = p;
+    } else if (ph == 4) {
       a.slot[p] = -1;
     } else {
       if (ph == 3) {
@@ -631,6 +577,22 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
     const int64_t key = (int64_t)t * TS + threadIdx.x;
     kbase[key] = atomicAdd(&a.count_next[key], cnt);
   }
+}
+
+// Particles deferred by k_step_tiled, stepped from their start state with the
+// FP64 routine for uncertified faces (a grid-stride loop over the list).
+template <typename R, bool EXACT, bool RK2>
+__global__ __launch_bounds__(kBlock) void k_step_deferred(StepArgs<R> a) {
+  const int64_t nd = *a.defer_count;
+  StepTally tally;
+  const PlaneSrc<R, false> src{a.g, a.corner, (const typename PlanesOf<R>::type *)a.planes, nullptr, 0, 0, 0};
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nd; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = a.defer_list[q];
+    PState<R> ps;
+    load_state(a, p, ps);
+    step_particle<R, EXACT, RK2>(a, src, p, ps, tally);
+  }
+  flush_tally(tally, a.counters);
 }
 
 // Particles idx[q0 .. q1) from global memory (particles without a cell).
@@ -765,7 +727,7 @@ __global__ __launch_bounds__(kBlock) void k_deposit_cells(int IM, int JM, int KM
                                                           const int32_t *__restrict__ idx, const R *__restrict__ w,
                                                           const R *__restrict__ vel, R *__restrict__ fi,
                                                           R *__restrict__ fj, R *__restrict__ fk) {
-  const int b = blockIdx.x;
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
   const int bi = b % xm.NTI, bj = (b / xm.NTI) % xm.NTJ, bk = b / (xm.NTI * xm.NTJ);
   const int l = threadIdx.x;
   const int i = bi * TX + (l % TX), j = bj * TY + ((l / TX) % TY), k = bk * TZ + (l / (TX * TY));

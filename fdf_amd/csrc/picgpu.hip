@@ -236,7 +236,8 @@ int picgpu_destroy(picgpu_ctx *c) {
                     &c->vel0, &c->cell_count, &c->cell_start, &c->pavg, &c->face[0], &c->face[1], &c->face[2],
                     &c->scan_tmp, &c->counters, &c->mlp_master, &c->mlp_bf16, &c->mlp_adam_m, &c->mlp_adam_v,
                     &c->mlp_grad, &c->mlp_partial, &c->mlp_x, &c->mlp_t, &c->mlp_y, &c->kcount,
-                    &c->kstart[0], &c->kstart[1], &c->kidx[0], &c->kidx[1], &c->kslot, &c->kbase, &c->planes};
+                    &c->kstart[0], &c->kstart[1], &c->kidx[0], &c->kidx[1], &c->kslot, &c->kbase, &c->planes,
+                    &c->defer};
   for (DevBuf *b : bufs) b->release();
   SwarmBufs *sws[] = {&c->sw, &c->alt};
   for (SwarmBufs *s : sws) {
@@ -830,6 +831,9 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, b
   PG_CHECK(c->kstart[nxt].ensure(sizeof(int32_t) * (size_t)(nkeys + 2)));
   PG_CHECK(c->kidx[nxt].ensure(sizeof(int32_t) * (size_t)(n > 0 ? n : 1)));
   PG_HIP(hipMemsetAsync(c->kcount.ptr, 0, sizeof(int32_t) * (size_t)(nkeys + 1), c->stream));
+  // defer list [n] and its length
+  PG_CHECK(c->defer.ensure(sizeof(int32_t) * (size_t)((n > 0 ? n : 1) + 1)));
+  PG_HIP(hipMemsetAsync(c->defer.as<int32_t>() + (n > 0 ? n : 1), 0, sizeof(int32_t), c->stream));
   const int32_t *idx = c->kidx[cur].as<int32_t>();
   const int32_t *kstart = c->kstart[cur].as<int32_t>();
   PG_CHECK(c->kbase.ensure(sizeof(int32_t) * (size_t)(nkeys > 0 ? nkeys : 1)));
@@ -860,16 +864,23 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, b
       a.count_next = c->kcount.as<int32_t>();
       a.slot = c->kslot.as<int32_t>();
       a.counters = c->counters.as<unsigned long long>();
+      a.defer_list = c->defer.as<int32_t>();
+      a.defer_count = c->defer.as<int32_t>() + (n > 0 ? n : 1);
       constexpr bool EX = sizeof(R) == sizeof(double);
       const char *name = EX ? "step_fused_fp64" : "step_fused_fp32";
       const char *name_list = EX ? "step_list_fp64" : "step_list_fp32";
       const int64_t u0 = c->kbegin_unloc, u1 = c->kend_unloc;
+      // deferred particles: at most the particles of the index; the kernel
+      // reads their number on the device
+      const int64_t dblocks = std::min<int64_t>(1024, nblocks(m));
       if (rk2) {
         PG_LAUNCH(c, name, nt, kBlock, (k_step_tiled<R, EX, true>), a, idx, kstart, kbase);
         if (u1 > u0) PG_LAUNCH(c, name_list, nblocks(u1 - u0), kBlock, (k_step_list<R, EX, true>), a, idx, u0, u1);
+        if (!EX) PG_LAUNCH(c, "step_deferred", dblocks, kBlock, (k_step_deferred<R, EX, true>), a);
       } else {
         PG_LAUNCH(c, name, nt, kBlock, (k_step_tiled<R, EX, false>), a, idx, kstart, kbase);
         if (u1 > u0) PG_LAUNCH(c, name_list, nblocks(u1 - u0), kBlock, (k_step_list<R, EX, false>), a, idx, u0, u1);
+        if (!EX) PG_LAUNCH(c, "step_deferred", dblocks, kBlock, (k_step_deferred<R, EX, false>), a);
       }
       return PICGPU_OK;
     });

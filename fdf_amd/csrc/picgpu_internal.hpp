@@ -137,6 +137,7 @@ struct picgpu_ctx {
   picgpu::DevBuf kidx[2];    // int32 [n] particle numbers in key order
   picgpu::DevBuf kslot;      // int32 [n] slot of each particle in its next bucket
   picgpu::DevBuf kbase;      // int32 [nkeys] a tile's base in each of its own buckets
+  picgpu::DevBuf defer;      // int32 [n] particles deferred to the FP64 kernel, then their number
   int kcur = 0;              // index in use
   bool keyed = false;        // kidx[kcur] / kstart[kcur] describe the current swarm
   int64_t kbegin_unloc = 0, kend_unloc = 0;  // range of bucket nkeys in kidx[kcur]

@@ -461,9 +461,9 @@ __device__ __forceinline__ bool plane_fp32(const CellPlanes32 &r, const float q[
   return fabsf(dd) > band;
 }
 
-__device__ __forceinline__ FilterResult eval_planes_fp32(const CellPlanes32 &r, const GridView &g, int i, int j, int k,
-                                                         const double p[3], double thresh_scale, float d[6]) {
-  FilterResult res{0, false};
+// FP32 distances; returns the faces (bit f: face f in evaluation order) that
+// the bound does not certify.
+__device__ __forceinline__ unsigned planes_fp32_distances(const CellPlanes32 &r, const double p[3], float d[6]) {
   const float q[3] = {(float)(p[0] - r.v0[0]), (float)(p[1] - r.v0[1]), (float)(p[2] - r.v0[2])};
   const float Q = fmaxf(fabsf(q[0]), fmaxf(fabsf(q[1]), fabsf(q[2])));
   const float u = 5.9604645e-08f;  // 2^-24
@@ -475,6 +475,13 @@ __device__ __forceinline__ FilterResult eval_planes_fp32(const CellPlanes32 &r, 
   need |= plane_fp32<3>(r, q, band, d) ? 0u : 8u;
   need |= plane_fp32<4>(r, q, band, d) ? 0u : 16u;
   need |= plane_fp32<5>(r, q, band, d) ? 0u : 32u;
+  return need;
+}
+
+__device__ __forceinline__ FilterResult eval_planes_fp32(const CellPlanes32 &r, const GridView &g, int i, int j, int k,
+                                                         const double p[3], double thresh_scale, float d[6]) {
+  FilterResult res{0, false};
+  const unsigned need = planes_fp32_distances(r, p, d);
   if (need) exact_fallback(g, i, j, k, p, thresh_scale, need, d, res);
   return res;
 }
