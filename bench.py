@@ -35,41 +35,42 @@ def log(*a):
 def algorithmic_bytes(kernel: str, n: int, g: dict, rsize: int) -> float:
     """Compulsory HBM bytes of one launch of `kernel` over n particles.
 
-    Particle record terms count each field read or written once; grid terms
-    count each node/cell/face array once.  Sorting scratch and re-reads are
-    excluded (implementation choices)."""
+    The fused step moves the particle state once in and once out (S bytes
+    each way: pos f64 24 + packed cell 4 + status 4, plus the FP64 velocity
+    24 in the FP64 variant) and reads each node's coordinates (24 B) and
+    corner velocity (3 reals) once.  Grid-record caches, sort scratch and
+    re-reads are implementation choices and are not counted."""
     Nn, Nc = g["nnode"], g["ncell"]
+    S = 32 if rsize == 4 else 56
     if kernel.startswith("step_fused"):
-        # read pos(24) vel(3r) cell(4) status(4); write pos(24) vel(3r) weights(3r) cell(4) status(4) slot(4);
-        # node coordinates (24 B) and corner velocity (3r) once
-        return n * (24 + 3 * rsize + 4 + 4 + 24 + 3 * rsize + 3 * rsize + 4 + 4 + 4) + Nn * (24 + 3 * rsize)
-    if kernel == "index_scatter":
-        return n * (4 + 4 + 4 + 4)
+        return n * 2 * S + Nn * (24 + 3 * rsize)
+    if kernel == "deposit_faces":
+        # weights and velocity of each particle once; count + 3 momentum per face
+        return n * 6 * rsize + g["nface"] * 4 * rsize
     if kernel == "gather_swarm":
         rec = 24 + 3 * rsize + 3 * rsize + 4 + 4 + 8 + rsize
         return n * (2 * rec + 4)
+    if kernel == "key_reindex":
+        return n * (4 + 4 + 4 + 4)
     if kernel.startswith("locate"):
-        # read pos(24) cell(4) status(4); write cell(4) status(4) weights(3r); nodes 24 B once
         return n * (24 + 4 + 4 + 4 + 4 + 3 * rsize) + Nn * 24
     if kernel == "interp":
         return n * (4 + 3 * rsize + 3 * rsize) + Nn * 3 * rsize
     if kernel == "center_to_corner":
         return Nc * 3 * rsize + Nn * 3 * rsize
-    if kernel == "deposit_faces":
-        # one direction: weights(1r)+velocity(3r)+pid(8) per particle, starts, faces x4
-        nf = g["nface_avg"]
-        return n * (rsize + 3 * rsize + 8) + Nc * 4 + nf * 4 * rsize
-    if kernel == "sort_scatter":
-        rec = 24 + 3 * rsize + 3 * rsize + 4 + 4 + 8 + rsize
-        return n * (2 * rec + 8)
-    if kernel in ("rk2_predict",):
-        return n * (24 + 3 * rsize + 4 + 24 + 3 * rsize + 24)
-    if kernel in ("rk2_correct",):
-        return n * (24 + 3 * rsize + 3 * rsize + 4 + 24 + 4)
-    if kernel == "compact":
-        rec = 24 + 3 * rsize + 3 * rsize + 4 + 4 + 8 + rsize
-        return n * (2 * rec + 8)
     return float("nan")
+
+
+def design_bytes(kernel: str, n: int, g: dict, rsize: int) -> float:
+    """Bytes the implementation itself moves per launch of the fused step:
+    particle fields (idx, status, cell, pos, velocity in; pos, velocity,
+    weights, cell, status, slot out), one face-plane record per cell (128 B
+    FP32, 304 B FP64) and the staged corner velocities."""
+    if not kernel.startswith("step_fused"):
+        return float("nan")
+    rec = 128 if rsize == 4 else 304
+    return n * ((4 + 4 + 4 + 24 + 3 * rsize) + (24 + 3 * rsize + 3 * rsize + 4 + 4 + 4)) + \
+        g["ncell"] * rec + g["nnode"] * 3 * rsize
 
 
 def step_bytes(n: int, g: dict, rsize: int, faces: bool) -> float:
@@ -78,6 +79,42 @@ def step_bytes(n: int, g: dict, rsize: int, faces: bool) -> float:
     c = 3 * rsize
     D = g["nface"] * 4 * rsize if faces else g["ncell"] * 2 * rsize
     return n * 2 * S + g["nnode"] * (24 + c) + g["ncell"] * c + g["nnode"] * c + D
+
+
+def measured_traffic(workload: str, variant: str, kernel: str) -> float | None:
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summaries (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration),
+    when one matches this workload and variant; otherwise None."""
+    import glob
+
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.txt"))):
+        kv = {}
+        with open(path) as f:
+            for line in f:
+                if ":" in line:
+                    k, v = line.split(":", 1)
+                    kv[k.strip()] = v.strip()
+        if kv.get("workload") == workload and kv.get("variant") == variant and kv.get("kernel") == kernel:
+            try:
+                return float(kv["traffic_bytes_per_launch"])
+            except (KeyError, ValueError):
+                return None
+    return None
+
+
+def calibration_copy(nbytes: int = 1 << 30) -> None:
+    """A plain device copy of known size, so that counter runs of this command
+    can scale FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    torch.cuda.empty_cache()
+    log(f"calibration copy: {nbytes} bytes read, {nbytes} bytes written")
 
 
 def make_inputs(cfg: int, scale: float, device: str):
@@ -171,8 +208,10 @@ def roofline_for(res: dict, d: dict, g: dict) -> dict:
     b = algorithmic_bytes(dom, int(n_avg), g, rsize)
     achieved = b / (kern[dom]["avg_ms"] * 1e-3) / 1e9
     return {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "algorithmic_bytes_per_launch": b, "avg_launch_ms": kern[dom]["avg_ms"],
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(d["workload_name"], res["variant"], dom),
+            "algorithmic_bytes_per_launch": b, "design_bytes_per_launch": design_bytes(dom, int(n_avg), g, rsize),
+            "avg_launch_ms": kern[dom]["avg_ms"], "particles_per_launch": n_avg,
             "share_of_step_time": kern[dom]["total_ms"] / (1e3 * res["seconds"])}
 
 
@@ -241,7 +280,9 @@ def main():
     torch.cuda.set_device(0)
     t0 = time.perf_counter()
     d = make_inputs(args.config, args.scale, dev)
+    d["workload_name"] = f"config{args.config} scale={args.scale:g}"
     torch.cuda.synchronize()
+    calibration_copy()
     log(f"inputs: config {args.config} {d['IM']}x{d['JM']}x{d['KM']} cells, {d['n']} particles, "
         f"{time.perf_counter() - t0:.1f}s")
     steps = args.steps if args.steps is not None else min(d["steps"], 100)

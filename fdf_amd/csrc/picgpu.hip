@@ -154,7 +154,7 @@ static int finish_grid(picgpu_ctx *c) {
   c->have_grid = true;
   c->have_planes = false;
   c->have_ucat = false;
-  c->sorted = c->keyed = false;
+  c->sorted = c->keyed = c->key_ordered = false;
   c->have_pavg = c->have_faces = false;
   return PICGPU_OK;
 }
@@ -332,7 +332,7 @@ int picgpu_init_particles_device(picgpu_ctx *c, int64_t n, const double *d_pos, 
   PG_HIP(hipSetDevice(c->device));
   PG_CHECK(ensure_swarm(c, c->sw, n));
   c->n = n;
-  c->sorted = c->keyed = false;
+  c->sorted = c->keyed = c->key_ordered = false;
   if (n == 0) return PICGPU_OK;
   int r = dispatch_real(c, [&](auto *tag) -> int {
     using R = std::remove_pointer_t<decltype(tag)>;
@@ -388,7 +388,7 @@ int picgpu_upload_particles(picgpu_ctx *c, int64_t n, const double *pos, const d
   PG_HIP(hipSetDevice(c->device));
   PG_CHECK(ensure_swarm(c, c->sw, n));
   c->n = n;
-  c->sorted = c->keyed = false;
+  c->sorted = c->keyed = c->key_ordered = false;
   if (n == 0) return PICGPU_OK;
   PG_HIP(hipMemcpy(c->sw.pos.ptr, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice));
   PG_HIP(hipMemcpy(c->sw.pid.ptr, pid, sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice));
@@ -486,7 +486,7 @@ static int do_reset_advect(picgpu_ctx *c, double dt, int reset, int advect) {
     using R = std::remove_pointer_t<decltype(tag)>;
     PG_LAUNCH(c, "reset_advect", nblocks(c->n), kBlock, k_reset_advect<R>, c->n, c->sw.pos.as<double>(),
               c->sw.vel.as<R>(), c->sw.status.as<int32_t>(), dt, reset, advect);
-    c->sorted = c->keyed = false;
+    c->sorted = c->keyed = c->key_ordered = false;
     return PICGPU_OK;
   });
 }
@@ -568,7 +568,7 @@ static int do_locate(picgpu_ctx *c, int skip_lost) {
   c->stats.fp64_fallbacks = (int64_t)h[C_FALLBACKS];
   c->stats.located = (int64_t)h[C_LOCATED];
   c->stats.lost = (int64_t)h[C_LOST];
-  c->sorted = c->keyed = false;
+  c->sorted = c->keyed = c->key_ordered = false;
   return PICGPU_OK;
 }
 
@@ -605,7 +605,7 @@ static int do_remove(picgpu_ctx *c, int check_lost, int check_box, int64_t *remo
   PG_CHECK(r);
   swap_swarm(c);
   c->n = total;
-  c->sorted = c->keyed = false;
+  c->sorted = c->keyed = c->key_ordered = false;
   return PICGPU_OK;
 }
 
@@ -684,7 +684,7 @@ static int do_sort(picgpu_ctx *c) {
     swap_swarm(c);
   }
   c->sorted = true;
-  c->keyed = false;
+  c->keyed = c->key_ordered = false;
   return PICGPU_OK;
 }
 
@@ -816,13 +816,19 @@ static int compact_by_key(picgpu_ctx *c, int64_t live) {
   }
   swap_swarm(c);
   c->n = live;
+  c->key_ordered = true;
   return PICGPU_OK;
 }
 
 static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, bool last) {
   const bool rk2 = (flags & PICGPU_STEP_RK2) != 0;
   PG_CHECK(ensure_planes(c));
-  if (first || !c->keyed) PG_CHECK(build_key_index(c));
+  if (first || !c->keyed) {
+    PG_CHECK(build_key_index(c));
+    // a swarm in arbitrary order (fresh upload, stage-wise calls) is moved
+    // into key order first, so that the tiles read their particles coalesced
+    if (!c->key_ordered) PG_CHECK(compact_by_key(c, c->kend_unloc));
+  }
   const TileMap tm = tile_map(c);
   const int64_t nt = ntiles(tm), nkeys = nt * TS;
   const int64_t n = c->n, m = c->n_live;  // m: entries of the current index

@@ -140,6 +140,7 @@ struct picgpu_ctx {
   picgpu::DevBuf defer;      // int32 [n] particles deferred to the FP64 kernel, then their number
   int kcur = 0;              // index in use
   bool keyed = false;        // kidx[kcur] / kstart[kcur] describe the current swarm
+  bool key_ordered = false;  // the swarm arrays are (nearly) in key order: compacted since the last change
   int64_t kbegin_unloc = 0, kend_unloc = 0;  // range of bucket nkeys in kidx[kcur]
 
   // per-cell / per-face outputs
