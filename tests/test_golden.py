@@ -1,0 +1,119 @@
+"""The oracle against the committed golden fixtures (tests/golden/), plus the
+facts each fixture must satisfy independently of the oracle (closed forms on
+the uniform grid, conservation, brute force on the warped grid)."""
+import os
+
+import numpy as np
+import pytest
+
+from fdf_amd import workloads as W
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load(name):
+    return np.load(os.path.join(GOLDEN, name))
+
+
+def test_walk_uniform_cfg1(oracle):
+    g = _load("walk_uniform_cfg1.npz")
+    IM, JM, KM = (int(v) for v in g["dims"])
+    coor = W.uniform_grid(IM, JM, KM)
+    cell, w, st, steps = oracle.locate(IM, JM, KM, coor, g["pos"])
+    np.testing.assert_array_equal(cell, g["cell"])
+    np.testing.assert_array_equal(st, g["status"])
+    np.testing.assert_array_equal(steps, g["steps"])
+    assert np.array_equal(w, g["weight"])
+    # closed form: inside the cube the owner is floor(x * dims), the walk from
+    # (0,0,0) takes max(|di|,|dj|,|dk|) + 1 iterations, weights are the
+    # fractional cell coordinates
+    pos = g["pos"]
+    inside = ((pos > 0) & (pos < 1)).all(1)
+    dims = np.array([IM, JM, KM])
+    frac = pos * dims - np.floor(pos * dims)
+    away = inside & ((frac > 1e-6) & (frac < 1 - 1e-6)).all(1)
+    exp = np.floor(pos * dims).astype(np.int64)
+    np.testing.assert_array_equal(g["cell"][away], exp[away])
+    np.testing.assert_array_equal(g["steps"][away], exp[away].max(1) + 1)
+    assert np.abs(g["weight"][away] - frac[away]).max() < 1e-9
+    assert (g["status"][~inside] == 3).all()
+
+
+def test_walk_warped_small(oracle):
+    g = _load("walk_warped_small.npz")
+    IM, JM, KM = (int(v) for v in g["dims"])
+    cell, w, st, steps = oracle.locate(IM, JM, KM, g["coor"], g["pos"])
+    np.testing.assert_array_equal(cell, g["cell"])
+    np.testing.assert_array_equal(st, g["status"])
+    np.testing.assert_array_equal(steps, g["steps"])
+    assert np.array_equal(w, g["weight"])
+    cell2, w2, st2, steps2 = oracle.locate(IM, JM, KM, g["coor"], g["pos"], cell=g["start"],
+                                           status=np.zeros(len(g["pos"]), np.int64))
+    np.testing.assert_array_equal(cell2, g["cell2"])
+    np.testing.assert_array_equal(st2, g["status2"])
+    np.testing.assert_array_equal(steps2, g["steps2"])
+    assert np.array_equal(w2, g["weight2"])
+    # every located cell accepts its point (brute-force check of the result)
+    c3 = g["coor"].reshape(KM + 1, JM + 1, IM + 1, 3)
+    off = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 1, 1), (1, 1, 1), (1, 0, 1), (0, 0, 1)]
+    for q in np.where(g["status"] == 1)[0][:300]:
+        c = g["cell"][q]
+        V = np.array([c3[c[2] + dk, c[1] + dj, c[0] + di] for di, dj, dk in off])
+        assert oracle.evaluate(V, g["pos"][q])[2] >= 0
+    # the warped grid makes some walks cycle: they end LOST at the cap
+    assert (g["status"] == 3).any() and (g["steps"][g["status"] == 3] == 1000).any()
+
+
+def test_interp_and_deposit_small(oracle):
+    g = _load("walk_warped_small.npz")
+    f = _load("interp_deposit_warped_small.npz")
+    IM, JM, KM = (int(v) for v in g["dims"])
+    corner = oracle.center_to_corner(IM, JM, KM, f["ucat"])
+    assert np.array_equal(corner, f["corner"])
+    vel = oracle.interp(IM, JM, KM, corner, g["cell"], g["weight"])
+    assert np.array_equal(vel, f["vel"])
+    count = oracle.count_cells(IM, JM, KM, g["cell"])
+    assert np.array_equal(count, f["count"])
+    assert count.sum() == (g["status"] == 1).sum()
+    pavg = oracle.scatter_P(IM, JM, KM, g["cell"], np.full(len(g["cell"]), 101325.0), f["pid"], count)
+    assert np.array_equal(pavg, f["pavg"])
+    assert set(np.unique(pavg)) <= {0.0, 101325.0}
+    faces = oracle.deposit_faces(IM, JM, KM, g["cell"], g["weight"], vel, f["pid"])
+    for a, b in zip(faces, (f["fi"], f["fj"], f["fk"])):
+        assert np.array_equal(a, b)
+    # conservation: every located particle spreads a total count of 1 over
+    # the two faces of each direction
+    nloc = (g["status"] == 1).sum()
+    for a in faces:
+        assert abs(a.reshape(-1, 4)[:, 0].sum() - nloc) < 1e-9 * nloc
+
+
+def test_steps_rk2_small(oracle):
+    g = _load("walk_warped_small.npz")
+    f = _load("interp_deposit_warped_small.npz")
+    s0 = _load("steps_rk2_warped_small.npz")
+    IM, JM, KM = (int(v) for v in g["dims"])
+    s = {k[3:]: s0[k].copy() for k in s0.files if k.startswith("in_")}
+    for _ in range(3):
+        oracle.step(IM, JM, KM, g["coor"], f["corner"], s, float(s0["dt"]), rk2=True)
+    for k in s:
+        assert np.array_equal(s[k], s0["out_" + k]), k
+    assert len(s["pid"]) < len(s0["in_pid"])  # some particles left or were lost
+
+
+def test_mlp_small(oracle):
+    m = _load("mlp_small.npz")
+    y = oracle.mlp_forward(m["params"], m["x"], round_bf16=True)
+    assert np.array_equal(y, m["y"])
+    loss, grad = oracle.mlp_loss_grad(m["params"], m["x"], m["t"], round_bf16=True)
+    assert loss == float(m["loss"]) and np.array_equal(grad, m["grad"])
+    loss0, grad0 = oracle.mlp_loss_grad(m["params"], m["x"], m["t"], round_bf16=False)
+    assert loss0 == float(m["loss_fp64"]) and np.array_equal(grad0, m["grad_fp64"])
+    # central differences on a few parameters of the unrounded network
+    rng = np.random.default_rng(0)
+    for i in rng.choice(len(m["params"]), 8, replace=False):
+        e = np.zeros_like(m["params"])
+        e[i] = 1e-6
+        lp, _ = oracle.mlp_loss_grad(m["params"] + e, m["x"], m["t"], round_bf16=False)
+        lm, _ = oracle.mlp_loss_grad(m["params"] - e, m["x"], m["t"], round_bf16=False)
+        assert (lp - lm) / 2e-6 == pytest.approx(grad0[i], rel=1e-4, abs=1e-8)
