@@ -414,6 +414,59 @@ __device__ __forceinline__ void flush_tally(StepTally &t, unsigned long long *co
 // in each bucket (kbase) is added when the next index is built.
 constexpr int32_t kLocalSlot = 0x40000000;
 
+// Results of the first OutCap<R> particles of a tile are staged in LDS and
+// written out together at the end of the workgroup, in index order: the
+// lanes finish their particles at different times, and writing each result
+// as it comes scatters partial cache lines over HBM.
+template <typename R>
+struct OutCap {
+  static constexpr int value = sizeof(R) == 4 ? 448 : 384;
+};
+enum OutKind : int8_t {
+  OUT_NONE = 0,   // deferred to k_step_deferred: nothing is written
+  OUT_SLOT = 1,   // removed before the step: slot -1 only
+  OUT_DEGEN = 2,  // degenerate face met: position, slot -1
+  OUT_LOST = 3,   // LOST: position, no cell, removed
+  OUT_OOB = 4,    // outside the bounding box: position, cell, weights, removed
+  OUT_LOC = 5     // located: position, cell, weights, velocity, slot
+};
+template <typename R>
+struct OutStage {
+  static constexpr int N = OutCap<R>::value;
+  double pos[N][3];
+  R vel[N][3];
+  R w[N][3];
+  int32_t cell[N];
+  int32_t slot[N];
+  int8_t kind[N];
+};
+
+template <typename R>
+__device__ __forceinline__ void emit_global(const StepArgs<R> &a, int64_t p, int kind, const double x[3],
+                                            int32_t cell, const R w[3], const R u[3], int32_t slot) {
+  if (kind == OUT_NONE) return;
+  if (kind != OUT_SLOT) {
+    a.pos[3 * p] = x[0];
+    a.pos[3 * p + 1] = x[1];
+    a.pos[3 * p + 2] = x[2];
+  }
+  if (kind >= OUT_LOST) {
+    a.cell[p] = cell;
+    a.status[p] = kind == OUT_LOC ? ST_LOCATED : ST_REMOVED;
+  }
+  if (kind >= OUT_OOB) {
+    a.w[3 * p] = w[0];
+    a.w[3 * p + 1] = w[1];
+    a.w[3 * p + 2] = w[2];
+  }
+  if (kind == OUT_LOC) {
+    a.vel[3 * p] = u[0];
+    a.vel[3 * p + 1] = u[1];
+    a.vel[3 * p + 2] = u[2];
+  }
+  a.slot[p] = slot;
+}
+
 // One workgroup per tile: stage the tile's grid data in LDS, then step the
 // tile's particles idx[kstart[t*TS] .. kstart[(t+1)*TS]), lane l taking
 // particles l, l + kBlock, ...  Each lane runs one loop over all its walk
@@ -427,8 +480,10 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
                                                        const int32_t *__restrict__ kstart,
                                                        int32_t *__restrict__ kbase) {
   static_assert(TS == kBlock, "one thread per key of the tile");
+  constexpr int CAP = OutCap<R>::value;
   __shared__ R sk[BN * 3];
   __shared__ int32_t scount[TS];
+  __shared__ OutStage<R> so;
   const int t = blockIdx.x;
   const int32_t s = kstart[(int64_t)t * TS], e = kstart[(int64_t)(t + 1) * TS];
   if (s == e) return;  // empty tile (uniform over the block)
@@ -466,7 +521,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
   int ci = 0, cj = 0, ck = 0, steps = 0, stot = 0;
   CycleGuard cyc;
   R wv[3] = {0, 0, 0};
-  int32_t fp = -1, fv = 0;  // deferred store of a slot taken in the global histogram
+  int32_t fq = -1, fv = 0;  // a slot taken in the global histogram, stored at the next one (or the end)
 
   auto begin = [&]() {
     // request the next particle's state, then set up the walk of p
@@ -484,6 +539,13 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
       ph = 3;
     else
       ph = RK2 ? 1 : 2;
+  };
+  // the slot of the particle at index position qq (staged or global)
+  auto put_slot = [&](int32_t qq, int32_t v) {
+    if (qq - s < CAP)
+      so.slot[qq - s] = v;
+    else
+      a.slot[idx[qq]] = v;
   };
   if (p >= 0) begin();
 
@@ -509,55 +571,62 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
       }
     }
     // particle p is done: outcome r at position xw (x0 when LOST at start)
+    int kind;
+    int32_t cell = -1, slot = -1;
     if (r == W_DEFER) {
       // stepped again from its start state by k_step_deferred
       a.defer_list[atomicAdd(a.defer_count, 1)] = p;
+      kind = OUT_NONE;
     } else if (ph == 4) {
-      a.slot[p] = -1;
+      kind = OUT_SLOT;
     } else {
       if (ph == 3) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) xw[c] = ps.x0[c];
       }
       tally.iters += stot + steps;
-      a.pos[3 * p] = xw[0];
-      a.pos[3 * p + 1] = xw[1];
-      a.pos[3 * p + 2] = xw[2];
       if (r == W_DEGENERATE) {
         tally.err = 1;
-        a.slot[p] = -1;
+        kind = OUT_DEGEN;
       } else if (r == W_LOST) {
         tally.lost++;
-        a.cell[p] = -1;
-        a.status[p] = ST_REMOVED;
-        a.slot[p] = -1;
+        kind = OUT_LOST;
       } else {
         const bool in = xw[0] >= a.bx0 && xw[0] <= a.bx1 && xw[1] >= a.by0 && xw[1] <= a.by1 && xw[2] >= a.bz0 &&
                         xw[2] <= a.bz1;
-        a.cell[p] = pack_cell(ci, cj, ck, a.g.IM, a.g.JM);
-        a.w[3 * p] = wv[0];
-        a.w[3 * p + 1] = wv[1];
-        a.w[3 * p + 2] = wv[2];
+        cell = pack_cell(ci, cj, ck, a.g.IM, a.g.JM);
         if (in) {
           tally.located++;
-          a.vel[3 * p] = ui[0];
-          a.vel[3 * p + 1] = ui[1];
-          a.vel[3 * p + 2] = ui[2];
-          a.status[p] = ST_LOCATED;
+          kind = OUT_LOC;
           const int32_t key = cell_key(a.tm, ci, cj, ck);
           if (key / TS == t) {
-            a.slot[p] = atomicAdd(&scount[key - t * TS], 1) | kLocalSlot;
+            slot = atomicAdd(&scount[key - t * TS], 1) | kLocalSlot;
           } else {
-            if (fp >= 0) a.slot[fp] = fv;
-            fp = p;
+            if (fq >= 0) put_slot(fq, fv);
+            fq = q;
             fv = atomicAdd(&a.count_next[key], 1);
           }
         } else {
           tally.oob++;
-          a.status[p] = ST_REMOVED;
-          a.slot[p] = -1;
+          kind = OUT_OOB;
         }
       }
+    }
+    const int32_t lq = q - s;
+    if (lq < CAP) {
+      so.kind[lq] = (int8_t)kind;
+      so.pos[lq][0] = xw[0];
+      so.pos[lq][1] = xw[1];
+      so.pos[lq][2] = xw[2];
+      so.cell[lq] = cell;
+      so.slot[lq] = slot;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        so.w[lq][c] = wv[c];
+        so.vel[lq][c] = ui[c];
+      }
+    } else {
+      emit_global(a, p, kind, xw, cell, wv, ui, slot);
     }
     // next particle
     q += B;
@@ -569,9 +638,19 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
       begin();
     }
   }
-  if (fp >= 0) a.slot[fp] = fv;
+  if (fq >= 0) put_slot(fq, fv);
   flush_tally(tally, a.counters);
   __syncthreads();
+  // staged results, in index order
+  const int32_t nst = e - s < CAP ? e - s : CAP;
+  for (int32_t lq = threadIdx.x; lq < nst; lq += B) {
+    const int kind = so.kind[lq];
+    if (kind == OUT_NONE) continue;
+    const double x[3] = {so.pos[lq][0], so.pos[lq][1], so.pos[lq][2]};
+    const R w[3] = {so.w[lq][0], so.w[lq][1], so.w[lq][2]};
+    const R u[3] = {so.vel[lq][0], so.vel[lq][1], so.vel[lq][2]};
+    emit_global(a, (int64_t)idx[s + lq], kind, x, so.cell[lq], w, u, so.slot[lq]);
+  }
   const int32_t cnt = scount[threadIdx.x];
   if (cnt > 0) {
     const int64_t key = (int64_t)t * TS + threadIdx.x;
