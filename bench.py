@@ -195,9 +195,85 @@ def run_variant(name: str, d: dict, steps: int, warmup: int, mlp: dict | None):
             kernels[k] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt}
     res["kernels"] = kernels
     g.timing(False)
+    # walk iterations per particle-step, from one more (untimed) step with statistics on
+    g.walk_stats(True)
+    g.AdvanceSimulation(d["dt"], 1, **flags)
+    st = g.stats()
+    res["walk_iterations_per_particle_step"] = walk_percentiles(st["walk_hist"])
+    res["walk_iterations_per_particle_step"]["mean"] = st["walk_iterations"] / max(1, int(st["walk_hist"].sum()))
+    g.walk_stats(False)
     g.close()
     log(f"{name}: {res['value']:.4g} particle-steps/s, {res['ms_per_step']:.3f} ms/step")
     return res
+
+
+def walk_percentiles(hist) -> dict:
+    """Median and 99th percentile of a walk-iteration histogram (last bin: 63 or
+    more, reported as 63), with the histogram's mean."""
+    import numpy as np
+
+    h = np.asarray(hist, np.float64)
+    n = h.sum()
+    if n <= 0:
+        return {"mean": None, "p50": None, "p99": None, "particles": 0}
+    c = np.cumsum(h) / n
+    b = np.arange(len(h))
+    return {"mean": float((h * b).sum() / n), "p50": int(np.searchsorted(c, 0.5)),
+            "p99": int(np.searchsorted(c, 0.99)), "max": int(b[h > 0].max()), "particles": int(n)}
+
+
+def precision_errors(d: dict, sample: int) -> dict:
+    """FP32-mode error against FP64 for one step from the same state.
+
+    The FP64 mode is bit-exact with the FP64 oracle (tests/), so it stands in
+    for the oracle at this size.  Both contexts start from the FP64 located
+    state of the first `sample` particles on the full grid and take one step
+    of the workload; particles are matched by PID."""
+    import numpy as np
+
+    from fdf_amd import _lib as L
+    from fdf_amd.swarm import PicGpu
+
+    IM, JM, KM = d["IM"], d["JM"], d["KM"]
+    pos = d["pos_dev"][:sample].contiguous()
+    flags = dict(rk2=d["rk2"], deposit_cells=False, deposit_faces=True)
+    out = {}
+    with PicGpu(precision="fp64") as g64:
+        g64.upload_grid_device(IM, JM, KM, d["coor"].data_ptr())
+        g64.upload_ucat_device(d["ucat"].data_ptr())
+        g64.init_particles_device(sample, pos.data_ptr())
+        g64.LocateAllParticlesInGrid()
+        g64.InterpolateAllFieldsToSwarm()
+        s0 = g64.download_particles()
+        g64.AdvanceSimulation(d["dt"], 1, **flags)
+        out["fp64"] = (g64.download_particles(), [g64.field(f) for f in (L.FIELD_IFACE, L.FIELD_JFACE, L.FIELD_KFACE)])
+    with PicGpu(precision="fp32") as g32:
+        g32.upload_grid_device(IM, JM, KM, d["coor"].data_ptr())
+        g32.upload_ucat_device(d["ucat"].data_ptr())
+        g32.upload_particles(s0["pos"], vel=s0["vel"], cell=s0["cell"], weight=s0["weight"], status=s0["status"],
+                             pid=s0["pid"])
+        g32.AdvanceSimulation(d["dt"], 1, **flags)
+        out["fp32"] = (g32.download_particles(), [g32.field(f) for f in (L.FIELD_IFACE, L.FIELD_JFACE, L.FIELD_KFACE)])
+    (a, fa), (b, fb) = out["fp32"], out["fp64"]
+    common, ia, ib = np.intersect1d(a["pid"], b["pid"], return_indices=True)
+    scale = float(d["ucat"].abs().max().item())
+    same = (a["cell"][ia] == b["cell"][ib]).all(1)
+    ok = same
+    cnt = np.concatenate([f.reshape(-1, 4)[:, 0] for f in fb])
+    mom = np.concatenate([f.reshape(-1, 4)[:, 1:].ravel() for f in fb])
+    cnt32 = np.concatenate([f.reshape(-1, 4)[:, 0] for f in fa])
+    mom32 = np.concatenate([f.reshape(-1, 4)[:, 1:].ravel() for f in fa])
+    return {
+        "sample": f"first {sample} particles of the workload on the full grid, one step from the FP64 located state",
+        "reference": "FP64 mode (bit-exact with the FP64 oracle)",
+        "particles_fp64": int(len(b["pid"])), "particles_fp32": int(len(a["pid"])),
+        "cell_mismatch": int((~same).sum()),
+        "weight_max_abs": float(np.abs(a["weight"][ia][ok] - b["weight"][ib][ok]).max()),
+        "velocity_max_rel": float(np.abs(a["vel"][ia][ok] - b["vel"][ib][ok]).max() / scale),
+        "position_max_abs": float(np.abs(a["pos"][ia] - b["pos"][ib]).max()),
+        "face_count_max_rel": float(np.abs(cnt32 - cnt).max() / max(1.0, np.abs(cnt).max())),
+        "face_momentum_max_rel": float(np.abs(mom32 - mom).max() / (scale * max(1.0, np.abs(cnt).max()))),
+    }
 
 
 def roofline_for(res: dict, d: dict, g: dict) -> dict:
@@ -265,6 +341,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 19)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--error-sample", type=int, default=1 << 22,
+                    help="particles of the FP32-vs-FP64 one-step error sample (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -276,6 +354,9 @@ def main():
     from fdf_amd import _lib
 
     _lib.load()
+    if args.config == 4:
+        main_relocate(args)
+        return
     dev = "cuda:0"
     torch.cuda.set_device(0)
     t0 = time.perf_counter()
@@ -345,6 +426,83 @@ def main():
     }
     if "fp64" in results and "fp32_mlp" in results:
         line["fp32_mlp_over_fp64"] = results["fp32_mlp"]["value"] / results["fp64"]["value"]
+    if args.error_sample > 0:
+        try:
+            line["fp32_error_vs_fp64"] = precision_errors(d, min(args.error_sample, d["n"]))
+            log("fp32 error vs fp64:", line["fp32_error_vs_fp64"])
+        except Exception as e:
+            log("precision error sample failed:", e)
+    print(json.dumps(line), flush=True)
+
+
+def run_relocate(d: dict, precision: str, use_mlp: bool, steps: int, warmup: int) -> dict:
+    """Config 4: every step re-locates the whole swarm from positions with no
+    cell (the start cell is the MLP guess, or (0,0,0) without it); only the
+    locate is timed."""
+    import torch
+
+    from fdf_amd import mlp as M
+    from fdf_amd.swarm import PicGpu
+
+    IM, JM, KM = d["IM"], d["JM"], d["KM"]
+    n = int(d["pos_dev"].shape[0])
+    res = {"precision": precision, "mlp": use_mlp}
+    with PicGpu(precision=precision, use_mlp_guess=False) as g:
+        g.upload_grid_device(IM, JM, KM, d["coor"].data_ptr())
+        g.upload_ucat_device(d["ucat"].data_ptr())
+        if use_mlp:
+            res["mlp_train"] = M.train_from_walk(g, d)
+        g.walk_stats(True)
+        secs, iters, hist = 0.0, 0, None
+        for s in range(warmup + steps):
+            g.init_particles_device(n, d["pos_dev"].data_ptr())  # cells discarded: no temporal coherence
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            g.LocateAllParticlesInGrid()
+            g.synchronize()
+            t1 = time.perf_counter()
+            if s >= warmup:
+                st = g.stats()
+                secs += t1 - t0
+                iters += st["walk_iterations"]
+                hist = st["walk_hist"] if hist is None else hist + st["walk_hist"]
+                res["located"], res["lost"] = st["located"], st["lost"]
+    res["locates_per_s"] = n * steps / secs
+    res["ms_per_locate"] = 1e3 * secs / steps
+    res["walk_iterations"] = walk_percentiles(hist)
+    # the histogram's last bin holds every walk of 63 or more iterations; the
+    # mean is taken from the exact total (walks that cycle count 1000, the cap)
+    res["walk_iterations"]["mean"] = iters / (n * steps)
+    log(f"relocate {precision} mlp={use_mlp}: {res['locates_per_s']:.4g} locates/s, "
+        f"mean walk {res['walk_iterations']['mean']:.2f}, lost {res['lost']}")
+    return res
+
+
+def main_relocate(args) -> None:
+    import torch
+
+    torch.cuda.set_device(0)
+    d = make_inputs(args.config, args.scale, "cuda:0")
+    log(f"inputs: config {args.config} {d['IM']}x{d['JM']}x{d['KM']} cells, {d['n']} particles")
+    steps = args.steps if args.steps is not None else d["steps"]
+    out = {}
+    for prec in ("fp32", "fp64"):
+        out[f"{prec}_mlp"] = run_relocate(d, prec, True, steps, args.warmup)
+        # without the guess the walks start at (0,0,0): long, so fewer repetitions
+        out[f"{prec}_plain"] = run_relocate(d, prec, False, min(steps, 2), 1)
+    head = out["fp32_mlp"]
+    line = {
+        "metric": "particle-locates/sec (fresh positions, walking search from the MLP guess), 1 GPU",
+        "value": head["locates_per_s"], "unit": "particle-locates/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_locate"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"config{args.config}", "grid_cells": [d["IM"], d["JM"], d["KM"]],
+                   "particles": d["n"], "scale": args.scale, "mode": "locate only, cells discarded each step"},
+        "walk_iterations_saved_by_guess": {
+            p: (out[f"{p}_plain"]["walk_iterations"]["mean"] or 0) - (out[f"{p}_mlp"]["walk_iterations"]["mean"] or 0)
+            for p in ("fp32", "fp64")},
+        "variants": out,
+    }
     print(json.dumps(line), flush=True)
 
 

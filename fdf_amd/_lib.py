@@ -60,6 +60,7 @@ class PicgpuStats(C.Structure):
         ("removed_lost", C.c_int64),
         ("removed_out_of_bounds", C.c_int64),
         ("particle_steps", C.c_int64),
+        ("walk_hist", C.c_int64 * 64),
     ]
 
 
@@ -98,6 +99,7 @@ _SIG = {
     "picgpu_grid_field_size": (C.c_int, [_P, C.c_int, C.POINTER(_I64)]),
     "picgpu_download_grid_field": (C.c_int, [_P, C.c_int, _P]),
     "picgpu_get_stats": (C.c_int, [_P, C.POINTER(PicgpuStats)]),
+    "picgpu_walk_stats_enable": (C.c_int, [_P, C.c_int]),
     "picgpu_timing_enable": (C.c_int, [_P, C.c_int]),
     "picgpu_timing_query": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(_I64)]),
     "picgpu_timing_reset": (C.c_int, [_P]),

@@ -70,7 +70,11 @@ struct StepArgs {
   unsigned long long *counters;
   int32_t *defer_list;   // [n] particles left to k_step_deferred
   int32_t *defer_count;  // entries of defer_list (zeroed by the caller)
+  unsigned long long *walk_hist;  // null, or [kCounterStripes][kWalkBins] walk-iteration histogram
 };
+
+constexpr int kWalkBins = 64;
+__device__ __forceinline__ int walk_bin(int n) { return n < kWalkBins - 1 ? n : kWalkBins - 1; }
 
 // corner velocities of cell (i,j,k) in the order of interpolation.c:990-999
 template <typename R>
@@ -352,6 +356,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<R> &a, const Src &s
     outcome = walk_one<R, EXACT>(src, x1, ci, cj, ck, a.max_traversal, a.thresh_scale, a.clamp, wv, s1, t.fb);
   }
   t.iters += s1 + s2;
+  if (a.walk_hist) atomicAdd(&a.walk_hist[(blockIdx.x & (kCounterStripes - 1)) * kWalkBins + walk_bin(s1 + s2)], 1ull);
   a.pos[3 * p] = x1[0];
   a.pos[3 * p + 1] = x1[1];
   a.pos[3 * p + 2] = x1[2];
@@ -484,6 +489,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
   __shared__ R sk[BN * 3];
   __shared__ int32_t scount[TS];
   __shared__ OutStage<R> so;
+  __shared__ int32_t shist[kWalkBins];
   const int t = blockIdx.x;
   const int32_t s = kstart[(int64_t)t * TS], e = kstart[(int64_t)(t + 1) * TS];
   if (s == e) return;  // empty tile (uniform over the block)
@@ -508,6 +514,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
     }
   }
   scount[threadIdx.x] = 0;
+  if (threadIdx.x < kWalkBins) shist[threadIdx.x] = 0;
   __syncthreads();
 
   StepTally tally;
@@ -585,6 +592,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
         for (int c = 0; c < 3; ++c) xw[c] = ps.x0[c];
       }
       tally.iters += stot + steps;
+      if (a.walk_hist) atomicAdd(&shist[walk_bin(stot + steps)], 1);
       if (r == W_DEGENERATE) {
         tally.err = 1;
         kind = OUT_DEGEN;
@@ -656,6 +664,9 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
     const int64_t key = (int64_t)t * TS + threadIdx.x;
     kbase[key] = atomicAdd(&a.count_next[key], cnt);
   }
+  if (a.walk_hist && threadIdx.x < kWalkBins && shist[threadIdx.x] > 0)
+    atomicAdd(&a.walk_hist[(blockIdx.x & (kCounterStripes - 1)) * kWalkBins + threadIdx.x],
+              (unsigned long long)shist[threadIdx.x]);
 }
 
 // Particles deferred by k_step_tiled, stepped from their start state with the

@@ -78,6 +78,7 @@ struct LocateArgs {
   int32_t *status;
   const int32_t *guess;  // may be null
   int32_t *steps_out;    // may be null
+  unsigned long long *walk_hist;  // null, or [kCounterStripes][64] walk-iteration histogram
   unsigned long long *counters;
   int max_traversal;
   double thresh_scale;
@@ -174,6 +175,8 @@ __global__ __launch_bounds__(kBlock) void k_locate(LocateArgs<R> a) {
         my_lost++;
       }
       if (a.steps_out) a.steps_out[p] = steps;
+      if (a.walk_hist)
+        atomicAdd(&a.walk_hist[(blockIdx.x & (kCounterStripes - 1)) * 64 + (steps < 63 ? steps : 63)], 1ull);
       my_iters += steps;
       my_max = my_max > (unsigned long long)steps ? my_max : (unsigned long long)steps;
       p = -1;

@@ -76,6 +76,28 @@ static int dispatch_real(picgpu_ctx *c, F &&f) {
 }
 
 static constexpr size_t kCounterBytes = sizeof(unsigned long long) * C_NUM * kCounterStripes;
+static constexpr size_t kHistBytes = sizeof(unsigned long long) * PICGPU_WALK_HIST_BINS * kCounterStripes;
+
+// walk-iteration histogram of the next locate / fused step (collect_walk_stats)
+static unsigned long long *hist_reset(picgpu_ctx *c) {
+  if (!c->opt.collect_walk_stats) return nullptr;
+  if (c->walk_hist.ensure(kHistBytes) != PICGPU_OK) return nullptr;
+  if (hipMemsetAsync(c->walk_hist.ptr, 0, kHistBytes, c->stream) != hipSuccess) return nullptr;
+  return c->walk_hist.as<unsigned long long>();
+}
+
+static int hist_read(picgpu_ctx *c) {
+  if (!c->opt.collect_walk_stats || !c->walk_hist.ptr) return PICGPU_OK;
+  std::vector<unsigned long long> h((size_t)PICGPU_WALK_HIST_BINS * kCounterStripes);
+  PG_HIP(hipMemcpyAsync(h.data(), c->walk_hist.ptr, kHistBytes, hipMemcpyDeviceToHost, c->stream));
+  PG_HIP(hipStreamSynchronize(c->stream));
+  for (int b = 0; b < PICGPU_WALK_HIST_BINS; ++b) {
+    int64_t v = 0;
+    for (int s = 0; s < kCounterStripes; ++s) v += (int64_t)h[(size_t)s * PICGPU_WALK_HIST_BINS + b];
+    c->stats.walk_hist[b] = v;
+  }
+  return PICGPU_OK;
+}
 
 static int counters_reset(picgpu_ctx *c) {
   PG_CHECK(c->counters.ensure(kCounterBytes));
@@ -237,7 +259,7 @@ int picgpu_destroy(picgpu_ctx *c) {
                     &c->scan_tmp, &c->counters, &c->mlp_master, &c->mlp_bf16, &c->mlp_adam_m, &c->mlp_adam_v,
                     &c->mlp_grad, &c->mlp_partial, &c->mlp_x, &c->mlp_t, &c->mlp_y, &c->kcount,
                     &c->kstart[0], &c->kstart[1], &c->kidx[0], &c->kidx[1], &c->kslot, &c->kbase, &c->planes,
-                    &c->defer};
+                    &c->defer, &c->walk_hist};
   for (DevBuf *b : bufs) b->release();
   SwarmBufs *sws[] = {&c->sw, &c->alt};
   for (SwarmBufs *s : sws) {
@@ -530,6 +552,7 @@ static int do_locate(picgpu_ctx *c, int skip_lost) {
       PG_CHECK(c->walk_steps.ensure(sizeof(int32_t) * (size_t)n));
       steps = c->walk_steps.as<int32_t>();
     }
+    unsigned long long *hist = hist_reset(c);
     const int64_t chunk = 256;  // particles per wavefront
     const int64_t waves = (n + chunk - 1) / chunk;
     const int64_t blocks = (waves * 64 + kBlock - 1) / kBlock;
@@ -546,6 +569,7 @@ static int do_locate(picgpu_ctx *c, int skip_lost) {
       a.status = c->sw.status.as<int32_t>();
       a.guess = guess;
       a.steps_out = steps;
+      a.walk_hist = hist;
       a.counters = c->counters.as<unsigned long long>();
       a.max_traversal = c->opt.max_traversal;
       a.thresh_scale = c->opt.distance_threshold;
@@ -568,6 +592,7 @@ static int do_locate(picgpu_ctx *c, int skip_lost) {
   c->stats.fp64_fallbacks = (int64_t)h[C_FALLBACKS];
   c->stats.located = (int64_t)h[C_LOCATED];
   c->stats.lost = (int64_t)h[C_LOST];
+  if (n > 0) PG_CHECK(hist_read(c));
   c->sorted = c->keyed = c->key_ordered = false;
   return PICGPU_OK;
 }
@@ -840,6 +865,7 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, b
   // defer list [n] and its length
   PG_CHECK(c->defer.ensure(sizeof(int32_t) * (size_t)((n > 0 ? n : 1) + 1)));
   PG_HIP(hipMemsetAsync(c->defer.as<int32_t>() + (n > 0 ? n : 1), 0, sizeof(int32_t), c->stream));
+  unsigned long long *hist = hist_reset(c);
   const int32_t *idx = c->kidx[cur].as<int32_t>();
   const int32_t *kstart = c->kstart[cur].as<int32_t>();
   PG_CHECK(c->kbase.ensure(sizeof(int32_t) * (size_t)(nkeys > 0 ? nkeys : 1)));
@@ -870,6 +896,7 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, b
       a.count_next = c->kcount.as<int32_t>();
       a.slot = c->kslot.as<int32_t>();
       a.counters = c->counters.as<unsigned long long>();
+      a.walk_hist = hist;
       a.defer_list = c->defer.as<int32_t>();
       a.defer_count = c->defer.as<int32_t>() + (n > 0 ? n : 1);
       constexpr bool EX = sizeof(R) == sizeof(double);
@@ -917,6 +944,7 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, b
   c->stats.lost = (int64_t)h[C_REMOVED_LOST];
   c->stats.walk_iterations = (int64_t)h[C_WALK_ITERS];
   c->stats.fp64_fallbacks = (int64_t)h[C_FALLBACKS];
+  if (hist) PG_CHECK(hist_read(c));
   const int32_t *nidx = c->kidx[nxt].as<int32_t>();
   const int32_t *nstart = c->kstart[nxt].as<int32_t>();
   int r = dispatch_real(c, [&](auto *tag) -> int {
@@ -1177,6 +1205,12 @@ int picgpu_get_stats(picgpu_ctx *c, picgpu_stats *st) {
   if (!c || !st) return PICGPU_E_ARG;
   *st = c->stats;
   st->n_particles = c->n;
+  return PICGPU_OK;
+}
+
+int picgpu_walk_stats_enable(picgpu_ctx *c, int on) {
+  if (!c) return PICGPU_E_ARG;
+  c->opt.collect_walk_stats = on ? 1 : 0;
   return PICGPU_OK;
 }
 

@@ -124,6 +124,7 @@ struct picgpu_ctx {
   picgpu::DevBuf rank;      // int32 [n] slot within the cell
   picgpu::DevBuf keep;      // int32 [n+1] keep flags / compaction offsets
   picgpu::DevBuf walk_steps;// int32 [n]
+  picgpu::DevBuf walk_hist; // uint64 [kCounterStripes][PICGPU_WALK_HIST_BINS]
   picgpu::DevBuf guess;     // int32 [n] MLP start cell
   picgpu::DevBuf pos0, vel0;// RK2 saved state
   bool sorted = false;      // particles sorted by cell, cell_start valid

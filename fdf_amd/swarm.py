@@ -133,7 +133,12 @@ class PicGpu:
     def stats(self) -> dict:
         st = L.PicgpuStats()
         L.check(self.lib.picgpu_get_stats(self.h, C.byref(st)))
-        return {f: getattr(st, f) for f, _ in L.PicgpuStats._fields_}
+        out = {f: getattr(st, f) for f, _ in L.PicgpuStats._fields_}
+        out["walk_hist"] = np.array(list(st.walk_hist), np.int64)
+        return out
+
+    def walk_stats(self, on: bool = True):
+        L.check(self.lib.picgpu_walk_stats_enable(self.h, int(on)))
 
     # ---- reference entry points ----
     def ResetAllParticleStatuses(self):

@@ -97,7 +97,12 @@ typedef struct {
   int64_t removed_lost;     /* by the last remove calls */
   int64_t removed_out_of_bounds;
   int64_t particle_steps;   /* particles processed, summed over picgpu_step steps */
+  /* collect_walk_stats=1: walk iterations per particle in the last fused step
+   * (both walks of an RK2 step) or the last locate; bin b counts particles
+   * with b iterations, the last bin those with PICGPU_WALK_HIST_BINS-1 or more */
+  int64_t walk_hist[64];
 } picgpu_stats;
+#define PICGPU_WALK_HIST_BINS 64
 
 /* ---- lifetime ---- */
 void picgpu_default_options(picgpu_options *opt);
@@ -156,6 +161,9 @@ int picgpu_download_grid_field(picgpu_ctx *ctx, int which, double *out);
 
 /* ---- statistics and timing ---- */
 int picgpu_get_stats(picgpu_ctx *ctx, picgpu_stats *st);
+/* Turns the walk statistics of picgpu_options.collect_walk_stats on or off
+ * (they cost a little time, so a benchmark enables them outside its timed steps). */
+int picgpu_walk_stats_enable(picgpu_ctx *ctx, int on);
 /* Kernel timing with HIP events on the library's stream: enable, then read the
  * summed milliseconds and launch count per kernel name. */
 int picgpu_timing_enable(picgpu_ctx *ctx, int on);

@@ -62,7 +62,9 @@ def test_golden_walk_uniform_cfg1(swarm_cls, precision):
         s.LocateAllParticlesInGrid()
         out = s.download_particles()
         steps = s.walk_steps()
+        hist = s.stats()["walk_hist"]
     _check_locate(out, g["cell"], g["status"], g["steps"], g["weight"], steps, precision)
+    np.testing.assert_array_equal(hist, np.bincount(np.minimum(g["steps"], 63), minlength=64))
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
@@ -132,9 +134,22 @@ def test_golden_rk2_steps(swarm_cls, precision, path):
         s.upload_ucat(f["ucat"])
         s.upload_particles(s0["in_pos"], vel=s0["in_vel"], cell=s0["in_cell"], weight=s0["in_weight"],
                            status=s0["in_status"], pid=s0["in_pid"], P=s0["in_P"])
-        s.AdvanceSimulation(float(s0["dt"]), 3, rk2=True, deposit_cells=True, deposit_faces=True,
+        s.AdvanceSimulation(float(s0["dt"]), 2, rk2=True, deposit_cells=True, deposit_faces=True,
                             staged=path == "staged")
+        s.walk_stats(True)
+        n_before = s.n
+        s.AdvanceSimulation(float(s0["dt"]), 1, rk2=True, deposit_cells=True, deposit_faces=True,
+                            staged=path == "staged")
+        st = s.stats()
         out = _by_pid(s.download_particles())
+    if path == "fused":
+        # the histogram of the last step covers every particle it stepped
+        h = st["walk_hist"]
+        assert h.sum() == n_before
+        # the last bin holds the walks of 63 or more iterations (cycling walks
+        # count MAX_TRAVERSAL = 1000, at most two walks per RK2 step)
+        below = (h[:63] * np.arange(63)).sum()
+        assert below + 63 * h[63] <= st["walk_iterations"] <= below + 2000 * h[63]
     o = np.argsort(s0["out_pid"])
     exp = {k[4:]: s0[k][o] for k in s0.files if k.startswith("out_")}
     if precision == "fp64":
