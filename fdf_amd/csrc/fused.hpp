@@ -805,54 +805,133 @@ __global__ __launch_bounds__(kBlock) void k_bucket_pid_order(int64_t nkeys, cons
 
 // Face deposition (no reference counterpart).  A particle in cell (i,j,k)
 // with weights (a1,a2,a3) adds (1-a1) to i-face i and a1 to i-face i+1, and
-// likewise along j and k; faces sum count and momentum.  One thread per
-// (i,j,k) of the extended range [0..IM]x[0..JM]x[0..KM] builds the lower
-// face of (i,j,k) in each direction: the lower neighbour's particles
-// (weight a) then the cell's own (weight 1-a), each in PID order (buckets
-// ordered by k_bucket_pid_order).
-// Threads are mapped to cells by 8x8x4 tiles of the extended range (xm).
+// likewise along j and k; faces sum count and momentum.  The lower face of
+// cell c in direction d sums the lower neighbour's particles (weight a_d),
+// then c's own (weight 1-a_d), each in PID order (buckets ordered by
+// k_bucket_pid_order), one accumulator running through both.
+//
+// One workgroup per 8x8x4 tile of the extended range [0..IM]x[0..JM]x[0..KM].
+// Pass 1: each thread runs its cell's particles once and leaves, per
+// direction, the accumulator state after the weight-a sums in LDS; 128
+// threads do the same, for one direction each, for the cells just below the
+// tile's lower faces.  Pass 2: each thread starts its three lower faces from
+// the lower neighbour's state and continues with its own particles (weight
+// 1-a).  Every particle is read once per pass, and the sums are those of the
+// one-accumulator order above.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_deposit_cells(int IM, int JM, int KM, TileMap tm, TileMap xm,
+__global__ __launch_bounds__(kBlock) void k_deposit_tiles(int IM, int JM, int KM, TileMap tm, TileMap xm,
                                                           const int32_t *__restrict__ kstart,
                                                           const int32_t *__restrict__ idx, const R *__restrict__ w,
                                                           const R *__restrict__ vel, R *__restrict__ fi,
                                                           R *__restrict__ fj, R *__restrict__ fk) {
+  static_assert(TS == kBlock, "one thread per cell of the tile");
+  constexpr int NH = TY * TZ + TX * TZ + TX * TY;  // cells below the tile's i-, j- and k-faces
+  constexpr bool COMP = sizeof(R) == 4;             // FP32 sums carry a compensation term
+  __shared__ R us[3][4][TS];                        // own cells: accumulator sum ...
+  __shared__ R uc[COMP ? 3 : 1][4][COMP ? TS : 1];  // ... and compensation
+  __shared__ R hs[4][NH], hc[4][COMP ? NH : 1];     // cells below the tile
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   const int bi = b % xm.NTI, bj = (b / xm.NTI) % xm.NTJ, bk = b / (xm.NTI * xm.NTJ);
   const int l = threadIdx.x;
-  const int i = bi * TX + (l % TX), j = bj * TY + ((l / TX) % TY), k = bk * TZ + (l / (TX * TY));
+  const int li = l % TX, lj = (l / TX) % TY, lk = l / (TX * TY);
+  const int i = bi * TX + li, j = bj * TY + lj, k = bk * TZ + lk;
+  const bool own = i < IM && j < JM && k < KM;
+  // weights and velocity of index position t
+  auto own_particle = [&](int32_t t, R a[3], R uu[3]) {
+    const int64_t q = idx[t];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      a[c] = w[3 * q + c];
+      uu[c] = vel[3 * q + c];
+    }
+  };
+  // pass 1, own cell: states after the weight-a sums, all three directions
+  {
+    Acc<R> acc[3][4];
+    if (own) {
+      const int32_t key = cell_key(tm, i, j, k);
+      const int32_t e = kstart[key + 1];
+      for (int32_t t = kstart[key]; t < e; ++t) {
+        R wa[3], uu[3];
+        own_particle(t, wa, uu);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const R a = wa[d];
+          acc[d][0].add(a);
+          acc[d][1].add(a * uu[0]);
+          acc[d][2].add(a * uu[1]);
+          acc[d][3].add(a * uu[2]);
+        }
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        us[d][v][l] = acc[d][v].s;
+        if constexpr (COMP) uc[d][v][l] = acc[d][v].c;
+      }
+  }
+  // pass 1, the cells below the tile (one direction each)
+  if (l < NH) {
+    int hi, hj, hk, d;
+    if (l < TY * TZ) {
+      d = 0, hi = bi * TX - 1, hj = bj * TY + l % TY, hk = bk * TZ + l / TY;
+    } else if (l < TY * TZ + TX * TZ) {
+      const int m = l - TY * TZ;
+      d = 1, hi = bi * TX + m % TX, hj = bj * TY - 1, hk = bk * TZ + m / TX;
+    } else {
+      const int m = l - TY * TZ - TX * TZ;
+      d = 2, hi = bi * TX + m % TX, hj = bj * TY + m / TX, hk = bk * TZ - 1;
+    }
+    Acc<R> acc[4];
+    if (hi >= 0 && hj >= 0 && hk >= 0 && hi < IM && hj < JM && hk < KM) {
+      const int32_t key = cell_key(tm, hi, hj, hk);
+      const int32_t e = kstart[key + 1];
+      for (int32_t t = kstart[key]; t < e; ++t) {
+        const int64_t q = idx[t];
+        const R a = w[3 * q + d];
+        acc[0].add(a);
+        acc[1].add(a * vel[3 * q]);
+        acc[2].add(a * vel[3 * q + 1]);
+        acc[3].add(a * vel[3 * q + 2]);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      hs[v][l] = acc[v].s;
+      if constexpr (COMP) hc[v][l] = acc[v].c;
+    }
+  }
+  __syncthreads();
   if (i > IM || j > JM || k > KM) return;
+  // pass 2: lower faces, from the lower neighbour's state through the own particles
   const bool has[3] = {j < JM && k < KM, i < IM && k < KM, i < IM && j < JM};
   Acc<R> acc[3][4];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    if (!has[d]) continue;
-    const int li = i - (d == 0), lj = j - (d == 1), lk = k - (d == 2);
-    if (li < 0 || lj < 0 || lk < 0) continue;
-    const int32_t key = cell_key(tm, li, lj, lk);
-    const int32_t e = kstart[key + 1];
-    for (int32_t t = kstart[key]; t < e; ++t) {
-      const int64_t q = idx[t];
-      const R a = w[3 * q + d];
-      acc[d][0].add(a);
-      acc[d][1].add(a * vel[3 * q]);
-      acc[d][2].add(a * vel[3 * q + 1]);
-      acc[d][3].add(a * vel[3 * q + 2]);
+    const bool inside = d == 0 ? li > 0 : d == 1 ? lj > 0 : lk > 0;
+    const int nl = l - (d == 0 ? 1 : d == 1 ? TX : TX * TY);  // lower neighbour in the tile
+    const int h = d == 0 ? lj + TY * lk : d == 1 ? TY * TZ + li + TX * lk : TY * TZ + TX * TZ + li + TX * lj;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      acc[d][v].s = inside ? us[d][v][nl] : hs[v][h];
+      if constexpr (COMP) acc[d][v].c = inside ? uc[d][v][nl] : hc[v][h];
     }
   }
-  if (i < IM && j < JM && k < KM) {
+  if (own) {
     const int32_t key = cell_key(tm, i, j, k);
     const int32_t e = kstart[key + 1];
     for (int32_t t = kstart[key]; t < e; ++t) {
-      const int64_t q = idx[t];
-      const R u0 = vel[3 * q], u1 = vel[3 * q + 1], u2 = vel[3 * q + 2];
+      R wa[3], uu[3];
+      own_particle(t, wa, uu);
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const R a = (R)1 - w[3 * q + d];
+        const R a = (R)1 - wa[d];
         acc[d][0].add(a);
-        acc[d][1].add(a * u0);
-        acc[d][2].add(a * u1);
-        acc[d][3].add(a * u2);
+        acc[d][1].add(a * uu[0]);
+        acc[d][2].add(a * uu[1]);
+        acc[d][3].add(a * uu[2]);
       }
     }
   }

@@ -958,7 +958,7 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, b
     if (flags & PICGPU_STEP_DEPOSIT_FACES) {
       for (int dir = 0; dir < 3; ++dir) PG_CHECK(c->face[dir].ensure(sizeof(R) * 4 * (size_t)nface(c, dir)));
       const TileMap xm{c->IM / TX + 1, c->JM / TY + 1, c->KM / TZ + 1};  // tiles over [0..IM]x[0..JM]x[0..KM]
-      PG_LAUNCH(c, "deposit_faces", ntiles(xm), kBlock, k_deposit_cells<R>, c->IM, c->JM, c->KM, tm, xm, nstart, nidx,
+      PG_LAUNCH(c, "deposit_faces", ntiles(xm), kBlock, k_deposit_tiles<R>, c->IM, c->JM, c->KM, tm, xm, nstart, nidx,
                 c->sw.w.as<R>(), c->sw.vel.as<R>(), c->face[0].as<R>(), c->face[1].as<R>(), c->face[2].as<R>());
       c->have_faces = true;
     }
