@@ -71,6 +71,8 @@ struct StepArgs {
   int32_t *defer_list;   // [n] particles left to k_step_deferred
   int32_t *defer_count;  // entries of defer_list (zeroed by the caller)
   unsigned long long *walk_hist;  // null, or [kCounterStripes][kWalkBins] walk-iteration histogram
+  int all_located;  // 1 after the first step of a call: every particle of the index is
+                    // ACTIVE_AND_LOCATED, statuses are neither read nor written
 };
 
 constexpr int kWalkBins = 64;
@@ -298,7 +300,7 @@ struct PState {
 
 template <typename R>
 __device__ __forceinline__ void load_state(const StepArgs<R> &a, int64_t p, PState<R> &s) {
-  s.st = a.status[p];
+  s.st = a.all_located ? ST_LOCATED : a.status[p];
   s.c0 = a.cell[p];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -457,7 +459,7 @@ __device__ __forceinline__ void emit_global(const StepArgs<R> &a, int64_t p, int
   }
   if (kind >= OUT_LOST) {
     a.cell[p] = cell;
-    a.status[p] = kind == OUT_LOC ? ST_LOCATED : ST_REMOVED;
+    if (!a.all_located) a.status[p] = kind == OUT_LOC ? ST_LOCATED : ST_REMOVED;
   }
   if (kind >= OUT_OOB) {
     a.w[3 * p] = w[0];
@@ -975,13 +977,19 @@ __global__ __launch_bounds__(kBlock) void k_count_from_keys(int IM, int JM, int 
   out[c] = kstart[key + 1] - kstart[key];
 }
 
-// Physical reorder: dst[q] = src[idx[q]] for q < m.
+// Physical reorder: dst[q] = src[idx[q]] for q < m.  Between the steps of one
+// call (GATHER_STEP) weights and statuses are not moved: the next step
+// recomputes the weights and every particle is ACTIVE_AND_LOCATED.  At the
+// end of a call (GATHER_LAST) weights move and statuses are set to
+// ACTIVE_AND_LOCATED.
+enum GatherMode : int { GATHER_ALL = 0, GATHER_STEP = 1, GATHER_LAST = 2 };
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_gather_swarm(int64_t m, const int32_t *__restrict__ idx, const double *pos,
-                                                         const R *vel, const R *w, const int32_t *cell,
-                                                         const int32_t *status, const int64_t *pid, const R *P,
-                                                         double *pos2, R *vel2, R *w2, int32_t *cell2,
-                                                         int32_t *status2, int64_t *pid2, R *P2) {
+__global__ __launch_bounds__(kBlock) void k_gather_swarm(int64_t m, int mode, const int32_t *__restrict__ idx,
+                                                         const double *pos, const R *vel, const R *w,
+                                                         const int32_t *cell, const int32_t *status,
+                                                         const int64_t *pid, const R *P, double *pos2, R *vel2,
+                                                         R *w2, int32_t *cell2, int32_t *status2, int64_t *pid2,
+                                                         R *P2) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= m) return;
   int64_t p = idx[q];
@@ -989,10 +997,11 @@ __global__ __launch_bounds__(kBlock) void k_gather_swarm(int64_t m, const int32_
   for (int c = 0; c < 3; ++c) {
     pos2[3 * q + c] = pos[3 * p + c];
     vel2[3 * q + c] = vel[3 * p + c];
-    w2[3 * q + c] = w[3 * p + c];
+    if (mode != GATHER_STEP) w2[3 * q + c] = w[3 * p + c];
   }
   cell2[q] = cell[p];
-  status2[q] = status[p];
+  if (mode == GATHER_ALL) status2[q] = status[p];
+  if (mode == GATHER_LAST) status2[q] = ST_LOCATED;
   pid2[q] = pid[p];
   P2[q] = P[p];
 }

@@ -823,13 +823,13 @@ static int build_key_index(picgpu_ctx *c) {
 
 // Moves the live particles into key order (the order of kidx[kcur]); the
 // index becomes the identity.
-static int compact_by_key(picgpu_ctx *c, int64_t live) {
+static int compact_by_key(picgpu_ctx *c, int64_t live, int mode) {
   PG_CHECK(ensure_swarm(c, c->alt, c->n));
   int32_t *idx = c->kidx[c->kcur].as<int32_t>();
   if (live > 0) {
     int r = dispatch_real(c, [&](auto *tag) -> int {
       using R = std::remove_pointer_t<decltype(tag)>;
-      PG_LAUNCH(c, "gather_swarm", nblocks(live), kBlock, k_gather_swarm<R>, live, idx, c->sw.pos.as<double>(),
+      PG_LAUNCH(c, "gather_swarm", nblocks(live), kBlock, k_gather_swarm<R>, live, mode, idx, c->sw.pos.as<double>(),
                 c->sw.vel.as<R>(), c->sw.w.as<R>(), c->sw.cell.as<int32_t>(), c->sw.status.as<int32_t>(),
                 c->sw.pid.as<int64_t>(), c->sw.P.as<R>(), c->alt.pos.as<double>(), c->alt.vel.as<R>(),
                 c->alt.w.as<R>(), c->alt.cell.as<int32_t>(), c->alt.status.as<int32_t>(), c->alt.pid.as<int64_t>(),
@@ -852,7 +852,7 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, b
     PG_CHECK(build_key_index(c));
     // a swarm in arbitrary order (fresh upload, stage-wise calls) is moved
     // into key order first, so that the tiles read their particles coalesced
-    if (!c->key_ordered) PG_CHECK(compact_by_key(c, c->kend_unloc));
+    if (!c->key_ordered) PG_CHECK(compact_by_key(c, c->kend_unloc, GATHER_ALL));
   }
   const TileMap tm = tile_map(c);
   const int64_t nt = ntiles(tm), nkeys = nt * TS;
@@ -897,6 +897,7 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, b
       a.slot = c->kslot.as<int32_t>();
       a.counters = c->counters.as<unsigned long long>();
       a.walk_hist = hist;
+      a.all_located = first ? 0 : 1;
       a.defer_list = c->defer.as<int32_t>();
       a.defer_count = c->defer.as<int32_t>() + (n > 0 ? n : 1);
       constexpr bool EX = sizeof(R) == sizeof(double);
@@ -967,7 +968,8 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, b
   PG_CHECK(r);
   c->steps_since_compact++;
   if (last || c->steps_since_compact >= kCompactEvery) {
-    PG_CHECK(compact_by_key(c, live));
+    // every survivor of a fused step is located
+    PG_CHECK(compact_by_key(c, live, last ? GATHER_LAST : GATHER_STEP));
     c->steps_since_compact = 0;
   }
   c->sorted = false;  // key order, not cell order
