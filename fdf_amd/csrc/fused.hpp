@@ -818,7 +818,8 @@ __global__ __launch_bounds__(kBlock) void k_bucket_pid_order(int64_t nkeys, cons
 // threads do the same, for one direction each, for the cells just below the
 // tile's lower faces.  Pass 2: each thread starts its three lower faces from
 // the lower neighbour's state and continues with its own particles (weight
-// 1-a).  Every particle is read once per pass, and the sums are those of the
+// 1-a).  A cell's first particles are read once, with independent loads, and
+// kept in registers for both passes; the sums are those of the
 // one-accumulator order above.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_deposit_tiles(int IM, int JM, int KM, TileMap tm, TileMap xm,
@@ -847,24 +848,39 @@ __global__ __launch_bounds__(kBlock) void k_deposit_tiles(int IM, int JM, int KM
       uu[c] = vel[3 * q + c];
     }
   };
+  // the first K particles of the own cell are read once, with independent
+  // loads, and kept in registers for both passes
+  constexpr int K = 4;
+  int32_t s_own = 0, n_own = 0;
+  if (own) {
+    const int32_t key = cell_key(tm, i, j, k);
+    s_own = kstart[key];
+    n_own = kstart[key + 1] - s_own;
+  }
+  R cw[K][3], cv[K][3];
+#pragma unroll
+  for (int u = 0; u < K; ++u)
+    if (u < n_own) own_particle(s_own + u, cw[u], cv[u]);
   // pass 1, own cell: states after the weight-a sums, all three directions
   {
     Acc<R> acc[3][4];
-    if (own) {
-      const int32_t key = cell_key(tm, i, j, k);
-      const int32_t e = kstart[key + 1];
-      for (int32_t t = kstart[key]; t < e; ++t) {
-        R wa[3], uu[3];
-        own_particle(t, wa, uu);
+    auto add_a = [&](const R wa[3], const R uu[3]) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const R a = wa[d];
-          acc[d][0].add(a);
-          acc[d][1].add(a * uu[0]);
-          acc[d][2].add(a * uu[1]);
-          acc[d][3].add(a * uu[2]);
-        }
+      for (int d = 0; d < 3; ++d) {
+        const R a = wa[d];
+        acc[d][0].add(a);
+        acc[d][1].add(a * uu[0]);
+        acc[d][2].add(a * uu[1]);
+        acc[d][3].add(a * uu[2]);
       }
+    };
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+      if (u < n_own) add_a(cw[u], cv[u]);
+    for (int32_t t = K; t < n_own; ++t) {
+      R wa[3], uu[3];
+      own_particle(s_own + t, wa, uu);
+      add_a(wa, uu);
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -889,8 +905,26 @@ __global__ __launch_bounds__(kBlock) void k_deposit_tiles(int IM, int JM, int KM
     Acc<R> acc[4];
     if (hi >= 0 && hj >= 0 && hk >= 0 && hi < IM && hj < JM && hk < KM) {
       const int32_t key = cell_key(tm, hi, hj, hk);
-      const int32_t e = kstart[key + 1];
-      for (int32_t t = kstart[key]; t < e; ++t) {
+      const int32_t sh = kstart[key], nh = kstart[key + 1] - sh;
+      R hw[K], hv[K][3];
+#pragma unroll
+      for (int u = 0; u < K; ++u)
+        if (u < nh) {
+          const int64_t q = idx[sh + u];
+          hw[u] = w[3 * q + d];
+          hv[u][0] = vel[3 * q];
+          hv[u][1] = vel[3 * q + 1];
+          hv[u][2] = vel[3 * q + 2];
+        }
+#pragma unroll
+      for (int u = 0; u < K; ++u)
+        if (u < nh) {
+          acc[0].add(hw[u]);
+          acc[1].add(hw[u] * hv[u][0]);
+          acc[2].add(hw[u] * hv[u][1]);
+          acc[3].add(hw[u] * hv[u][2]);
+        }
+      for (int32_t t = sh + K; t < sh + nh; ++t) {
         const int64_t q = idx[t];
         const R a = w[3 * q + d];
         acc[0].add(a);
@@ -921,21 +955,23 @@ __global__ __launch_bounds__(kBlock) void k_deposit_tiles(int IM, int JM, int KM
       if constexpr (COMP) acc[d][v].c = inside ? uc[d][v][nl] : hc[v][h];
     }
   }
-  if (own) {
-    const int32_t key = cell_key(tm, i, j, k);
-    const int32_t e = kstart[key + 1];
-    for (int32_t t = kstart[key]; t < e; ++t) {
-      R wa[3], uu[3];
-      own_particle(t, wa, uu);
+  auto add_1ma = [&](const R wa[3], const R uu[3]) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const R a = (R)1 - wa[d];
-        acc[d][0].add(a);
-        acc[d][1].add(a * uu[0]);
-        acc[d][2].add(a * uu[1]);
-        acc[d][3].add(a * uu[2]);
-      }
+    for (int d = 0; d < 3; ++d) {
+      const R a = (R)1 - wa[d];
+      acc[d][0].add(a);
+      acc[d][1].add(a * uu[0]);
+      acc[d][2].add(a * uu[1]);
+      acc[d][3].add(a * uu[2]);
     }
+  };
+#pragma unroll
+  for (int u = 0; u < K; ++u)
+    if (u < n_own) add_1ma(cw[u], cv[u]);
+  for (int32_t t = K; t < n_own; ++t) {
+    R wa[3], uu[3];
+    own_particle(s_own + t, wa, uu);
+    add_1ma(wa, uu);
   }
   R *out[3] = {fi + 4 * (((int64_t)k * JM + j) * (IM + 1) + i), fj + 4 * (((int64_t)k * (JM + 1) + j) * IM + i),
                fk + 4 * (((int64_t)k * JM + j) * IM + i)};
