@@ -78,39 +78,6 @@ struct StepArgs {
 constexpr int kWalkBins = 64;
 __device__ __forceinline__ int walk_bin(int n) { return n < kWalkBins - 1 ? n : kWalkBins - 1; }
 
-// corner velocities of cell (i,j,k) in the order of interpolation.c:990-999
-template <typename R>
-__device__ __forceinline__ void trilinear_vals(const R C[8][3], R a1, R a2, R a3, R out[3]) {
-  a1 = ((R)0 < ((R)1 < a1 ? (R)1 : a1)) ? ((R)1 < a1 ? (R)1 : a1) : (R)0;
-  a2 = ((R)0 < ((R)1 < a2 ? (R)1 : a2)) ? ((R)1 < a2 ? (R)1 : a2) : (R)0;
-  a3 = ((R)0 < ((R)1 < a3 ? (R)1 : a3)) ? ((R)1 < a3 ? (R)1 : a3) : (R)0;
-  const R oa1 = (R)1 - a1, oa2 = (R)1 - a2, oa3 = (R)1 - a3;
-  R wc[8];
-  wc[0] = oa1 * oa2 * oa3;
-  wc[1] = a1 * oa2 * oa3;
-  wc[2] = oa1 * a2 * oa3;
-  wc[3] = a1 * a2 * oa3;
-  wc[4] = oa1 * oa2 * a3;
-  wc[5] = a1 * oa2 * a3;
-  wc[6] = oa1 * a2 * a3;
-  wc[7] = a1 * a2 * a3;
-  R sw = 0, ax = 0, ay = 0, az = 0;
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    ax += wc[c] * C[c][0];
-    ay += wc[c] * C[c][1];
-    az += wc[c] * C[c][2];
-    sw += wc[c];
-  }
-  if (sw > (R)1.0e-14) {
-    out[0] = ax / sw;
-    out[1] = ay / sw;
-    out[2] = az / sw;
-  } else {
-    out[0] = out[1] = out[2] = (R)0;
-  }
-}
-
 // Corner velocities of cell (i,j,k) from global memory (order of
 // interpolation.c:990-999).
 template <typename R>
@@ -194,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_planes64(GridView g, double thr
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= (int64_t)g.IM * g.JM * g.KM) return;
   int i, j, k;
-  unpack_cell((int32_t)c, g.IM, g.JM, i, j, k);
+  unpack_cell((int32_t)c, cell_idx(g), i, j, k);
   CellPlanes64 r;
   cell_planes_exact(g, i, j, k, thresh_scale, r);
   out[c] = r;
@@ -205,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_planes32(GridView g, double thr
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= (int64_t)g.IM * g.JM * g.KM) return;
   int i, j, k;
-  unpack_cell((int32_t)c, g.IM, g.JM, i, j, k);
+  unpack_cell((int32_t)c, cell_idx(g), i, j, k);
   CellPlanes64 r;
   cell_planes_exact(g, i, j, k, thresh_scale, r);
   const double *v0 = g.coor + 3 * cell_base_node(g, i, j, k);
@@ -321,7 +288,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<R> &a, const Src &s
   const double *x0 = ps.x0;
   const R *u0 = ps.u0;
   int ci = -1, cj = -1, ck = -1;
-  if (ps.c0 >= 0) unpack_cell(ps.c0, a.g.IM, a.g.JM, ci, cj, ck);
+  if (ps.c0 >= 0) unpack_cell(ps.c0, cell_idx(a.g), ci, cj, ck);
   double x1[3];
   R wv[3];  // set by a walk that locates
   int outcome;
@@ -537,7 +504,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
     if (pn >= 0) load_state(a, pn, pnext);
     pnn = q + 2 * B < e ? idx[q + 2 * B] : -1;
     ci = cj = ck = 0;
-    if (ps.c0 >= 0) unpack_cell(ps.c0, a.g.IM, a.g.JM, ci, cj, ck);
+    if (ps.c0 >= 0) unpack_cell(ps.c0, cell_idx(a.g), ci, cj, ck);
     steps = stot = 0;
     cyc = CycleGuard{};
 #pragma unroll
@@ -745,7 +712,7 @@ __global__ __launch_bounds__(kBlock) void k_key_scatter(int64_t n, const int32_t
 __global__ __launch_bounds__(kBlock) void k_key_reindex(int64_t m, const int32_t *__restrict__ idx_cur,
                                                         const int32_t *__restrict__ cell,
                                                         const int32_t *__restrict__ slot,
-                                                        const int32_t *__restrict__ kbase, int IM, int JM, TileMap tm,
+                                                        const int32_t *__restrict__ kbase, CellIdx cx, TileMap tm,
                                                         const int32_t *__restrict__ kstart,
                                                         int32_t *__restrict__ idx_next) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -754,7 +721,7 @@ __global__ __launch_bounds__(kBlock) void k_key_reindex(int64_t m, const int32_t
   int32_t s = slot[p];
   if (s < 0) return;
   int i, j, k;
-  unpack_cell(cell[p], IM, JM, i, j, k);
+  unpack_cell(cell[p], cx, i, j, k);
   const int32_t key = cell_key(tm, i, j, k);
   if (s & kLocalSlot) s = kbase[key] + (s & (kLocalSlot - 1));
   idx_next[kstart[key] + s] = p;

@@ -17,6 +17,22 @@ __device__ __forceinline__ void unpack_cell(int32_t c, int IM, int JM, int &i, i
   j = r % JM;
   k = r / JM;
 }
+
+// The same decoding without integer division, given rI = fl(1/IM) and
+// rIJ = fl(1/(IM*JM)) in FP64: (c + 0.5) * rIJ lies within (c/(IM*JM)) * 2^-51
+// of (c + 0.5)/(IM*JM), which is at least 0.5/(IM*JM) from an integer, so the
+// truncation is floor(c / (IM*JM)) for every c < 2^31 (likewise for j).
+struct CellIdx {
+  int IM, JM;
+  double rI, rIJ;
+};
+__device__ __forceinline__ void unpack_cell(int32_t c, const CellIdx &x, int &i, int &j, int &k) {
+  k = (int)(((double)c + 0.5) * x.rIJ);
+  const int r = c - k * (x.IM * x.JM);
+  j = (int)(((double)r + 0.5) * x.rI);
+  i = r - j * x.IM;
+}
+__device__ __forceinline__ CellIdx cell_idx(const GridView &g) { return CellIdx{g.IM, g.JM, g.rI, g.rIJ}; }
 __device__ __forceinline__ int32_t pack_cell(int i, int j, int k, int IM, int JM) {
   return (int32_t)(i + IM * (j + JM * k));
 }
@@ -117,7 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_locate(LocateArgs<R> a) {
             int32_t c = a.cell[q];
             if (c < 0 && a.guess) c = a.guess[q];
             if (c >= 0) {
-              unpack_cell(c, a.g.IM, a.g.JM, ci, cj, ck);
+              unpack_cell(c, cell_idx(a.g), ci, cj, ck);
             } else {
               ci = 0;  // (info.xs, info.ys, info.zs) on one rank, walkingsearch.c:611-616
               cj = 0;
@@ -249,9 +265,13 @@ __global__ __launch_bounds__(kBlock) void k_center_to_corner(int IM, int JM, int
 // Trilinear interpolation (interpolation.c:839-864, 957-1042) per particle
 // (interpolation.c:1341-1385): a cell outside the grid gives zero velocity.
 // ---------------------------------------------------------------------------
+// Trilinear interpolation from the corner values C (interpolation.c:957-1042
+// with the weights of :839-864): clamp to [0,1], the eight corner weights in
+// the order of :851-858 / 990-999, accumulate corner by corner, divide by the
+// weight sum when it exceeds 1e-14.  FP64 mode: the reference operations.
+// FP32 mode: fused multiply-adds and one hardware reciprocal.
 template <typename R>
-__device__ __forceinline__ void trilinear(int IM, int JM, const R *__restrict__ corner, int i, int j, int k, R a1,
-                                          R a2, R a3, R out[3]) {
+__device__ __forceinline__ void trilinear_vals(const R C[8][3], R a1, R a2, R a3, R out[3]) {
   a1 = ((R)0 < ((R)1 < a1 ? (R)1 : a1)) ? ((R)1 < a1 ? (R)1 : a1) : (R)0;
   a2 = ((R)0 < ((R)1 < a2 ? (R)1 : a2)) ? ((R)1 < a2 ? (R)1 : a2) : (R)0;
   a3 = ((R)0 < ((R)1 < a3 ? (R)1 : a3)) ? ((R)1 < a3 ? (R)1 : a3) : (R)0;
@@ -265,25 +285,50 @@ __device__ __forceinline__ void trilinear(int IM, int JM, const R *__restrict__ 
   wc[5] = a1 * oa2 * a3;
   wc[6] = oa1 * a2 * a3;
   wc[7] = a1 * a2 * a3;
-  const int64_t sy = IM + 1, sz = (int64_t)(JM + 1) * (IM + 1);
-  const int64_t b = (int64_t)k * sz + (int64_t)j * sy + i;
-  const int64_t idx[8] = {b, b + 1, b + sy, b + sy + 1, b + sz, b + sz + 1, b + sz + sy, b + sz + sy + 1};
   R sw = 0, ax = 0, ay = 0, az = 0;
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
-    const R *f = corner + 3 * idx[c];
-    ax += wc[c] * f[0];
-    ay += wc[c] * f[1];
-    az += wc[c] * f[2];
+    if constexpr (sizeof(R) == 4) {
+      ax = __builtin_fmaf(wc[c], C[c][0], ax);
+      ay = __builtin_fmaf(wc[c], C[c][1], ay);
+      az = __builtin_fmaf(wc[c], C[c][2], az);
+    } else {
+      ax += wc[c] * C[c][0];
+      ay += wc[c] * C[c][1];
+      az += wc[c] * C[c][2];
+    }
     sw += wc[c];
   }
   if (sw > (R)1.0e-14) {
-    out[0] = ax / sw;
-    out[1] = ay / sw;
-    out[2] = az / sw;
+    if constexpr (sizeof(R) == 4) {
+      const R rs = __builtin_amdgcn_rcpf(sw);
+      out[0] = ax * rs;
+      out[1] = ay * rs;
+      out[2] = az * rs;
+    } else {
+      out[0] = ax / sw;
+      out[1] = ay / sw;
+      out[2] = az / sw;
+    }
   } else {
     out[0] = out[1] = out[2] = (R)0;
   }
+}
+
+template <typename R>
+__device__ __forceinline__ void trilinear(int IM, int JM, const R *__restrict__ corner, int i, int j, int k, R a1,
+                                          R a2, R a3, R out[3]) {
+  const int64_t sy = IM + 1, sz = (int64_t)(JM + 1) * (IM + 1);
+  const int64_t b = (int64_t)k * sz + (int64_t)j * sy + i;
+  R C[8][3];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int64_t q = b + (c & 1) + ((c >> 1) & 1) * sy + ((c >> 2) & 1) * sz;
+    C[c][0] = corner[3 * q];
+    C[c][1] = corner[3 * q + 1];
+    C[c][2] = corner[3 * q + 2];
+  }
+  trilinear_vals<R>(C, a1, a2, a3, out);
 }
 
 template <typename R>

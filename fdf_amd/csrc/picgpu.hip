@@ -143,7 +143,10 @@ static int need_grid(picgpu_ctx *c) {
   return PICGPU_OK;
 }
 
-static GridView grid_view(picgpu_ctx *c) { return GridView{c->coor.as<double>(), c->IM, c->JM, c->KM}; }
+static GridView grid_view(picgpu_ctx *c) {
+  return GridView{c->coor.as<double>(), c->IM, c->JM, c->KM, c->IM > 0 ? 1.0 / c->IM : 0.0,
+                  c->IM > 0 && c->JM > 0 ? 1.0 / ((double)c->IM * (double)c->JM) : 0.0};
+}
 
 // Padded bounding box of all nodes (grid.c:762-883 on one rank) and state reset
 // after a grid upload.
@@ -923,7 +926,8 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, b
   PG_CHECK(scan_exclusive_i32(c, c->kcount.as<int32_t>(), c->kstart[nxt].as<int32_t>(), nkeys + 1));
   if (m > 0)
     PG_LAUNCH(c, "key_reindex", nblocks(m), kBlock, k_key_reindex, m, idx, c->sw.cell.as<int32_t>(),
-              c->kslot.as<int32_t>(), kbase, c->IM, c->JM, tm, c->kstart[nxt].as<int32_t>(), c->kidx[nxt].as<int32_t>());
+              c->kslot.as<int32_t>(), kbase, CellIdx{c->IM, c->JM, grid_view(c).rI, grid_view(c).rIJ}, tm,
+              c->kstart[nxt].as<int32_t>(), c->kidx[nxt].as<int32_t>());
   if (m > 0 && (flags & (PICGPU_STEP_DEPOSIT_CELLS | PICGPU_STEP_DEPOSIT_FACES)))
     PG_LAUNCH(c, "bucket_pid_order", nblocks(nkeys), kBlock, k_bucket_pid_order, nkeys, c->kstart[nxt].as<int32_t>(),
               c->kidx[nxt].as<int32_t>(), c->sw.pid.as<int64_t>());

@@ -46,6 +46,7 @@ __device__ __constant__ static const int8_t kFaceSlot[6] = {F_BACK, F_FRONT, F_B
 struct GridView {
   const double *coor;  // [(KM+1)(JM+1)(IM+1)][3]
   int IM, JM, KM;
+  double rI = 0.0, rIJ = 0.0;  // 1/IM and 1/(IM*JM) in FP64 (cell decoding, kernels.hpp)
 };
 
 // node index of vertex v of cell (i,j,k) (vertex order walkingsearch.c:537-544)
@@ -207,9 +208,16 @@ __device__ __forceinline__ void weights_from_distances(R d[6], R clamp, R w[3]) 
 #pragma unroll
   for (int f = 0; f < 6; ++f)
     if (d[f] <= clamp) d[f] = clamp;
-  w[0] = d[F_LEFT] / (d[F_LEFT] + d[F_RIGHT]);
-  w[1] = d[F_BOTTOM] / (d[F_BOTTOM] + d[F_TOP]);
-  w[2] = d[F_BACK] / (d[F_FRONT] + d[F_BACK]);
+  if constexpr (sizeof(R) == 4) {
+    // FP32 mode: hardware reciprocal (1 ulp), weights within a few ulp
+    w[0] = d[F_LEFT] * __builtin_amdgcn_rcpf(d[F_LEFT] + d[F_RIGHT]);
+    w[1] = d[F_BOTTOM] * __builtin_amdgcn_rcpf(d[F_BOTTOM] + d[F_TOP]);
+    w[2] = d[F_BACK] * __builtin_amdgcn_rcpf(d[F_FRONT] + d[F_BACK]);
+  } else {
+    w[0] = d[F_LEFT] / (d[F_LEFT] + d[F_RIGHT]);
+    w[1] = d[F_BOTTOM] / (d[F_BOTTOM] + d[F_TOP]);
+    w[2] = d[F_BACK] / (d[F_FRONT] + d[F_BACK]);
+  }
 }
 
 // ---------------------------------------------------------------------------
