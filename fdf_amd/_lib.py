@@ -19,6 +19,8 @@ ERRORS = {
     -4: "PICGPU_E_HIP",
     -5: "PICGPU_E_NOMEM",
     -6: "PICGPU_E_UNSUPPORTED",
+    -7: "PICGPU_E_SINGULAR",
+    -8: "PICGPU_E_GRID",
 }
 
 FP64, FP32 = 0, 1
@@ -29,6 +31,13 @@ FIELD_P_AVERAGE = 1
 FIELD_CORNER_VELOCITY = 2
 FIELD_IFACE, FIELD_JFACE, FIELD_KFACE = 3, 4, 5
 FIELD_IFACE_VOLFRAC, FIELD_JFACE_VOLFRAC, FIELD_KFACE_VOLFRAC = 6, 7, 8
+FIELD_UCAT, FIELD_UCONT, FIELD_CSI, FIELD_ETA, FIELD_ZET, FIELD_AJ = 9, 10, 11, 12, 13, 14
+
+# boundary faces, types and handlers (common.h:202-222)
+FACE_NAMES = ("-Xi", "+Xi", "-Eta", "+Eta", "-Zeta", "+Zeta")
+BCTYPE_UNDEFINED, BCTYPE_NOGRAD, BCTYPE_WALL, BCTYPE_SYMMETRY, BCTYPE_INLET, BCTYPE_OUTLET = range(6)
+BC_NOGRAD_COPY_GHOST, BC_WALL_NOSLIP, BC_WALL_MOVING, BC_SYMMETRY_PLANE = 1, 2, 3, 4
+BC_INLET_PARABOLIC, BC_INLET_CONSTANT_VELOCITY, BC_OUTLET_CONSERVATION = 5, 6, 9
 
 STEP_RK2 = 0x1
 STEP_DEPOSIT_CELLS = 0x2
@@ -61,6 +70,15 @@ class PicgpuStats(C.Structure):
         ("removed_out_of_bounds", C.c_int64),
         ("particle_steps", C.c_int64),
         ("walk_hist", C.c_int64 * 64),
+    ]
+
+
+class PicgpuBoundaryConfig(C.Structure):
+    _fields_ = [
+        ("type", C.c_int * 6),
+        ("handler", C.c_int * 6),
+        ("velocity", (C.c_double * 3) * 6),
+        ("vmax", C.c_double * 6),
     ]
 
 
@@ -103,6 +121,16 @@ _SIG = {
     "picgpu_timing_enable": (C.c_int, [_P, C.c_int]),
     "picgpu_timing_query": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(_I64)]),
     "picgpu_timing_reset": (C.c_int, [_P]),
+    "picgpu_default_boundary_config": (None, [C.POINTER(PicgpuBoundaryConfig)]),
+    "picgpu_parse_boundary_file": (C.c_int, [C.c_char_p, C.POINTER(PicgpuBoundaryConfig)]),
+    "picgpu_compute_metrics": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "picgpu_boundary_create": (C.c_int, [_P, C.POINTER(PicgpuBoundaryConfig)]),
+    "picgpu_upload_ucont": (C.c_int, [_P, _P]),
+    "picgpu_upload_ucont_device": (C.c_int, [_P, _P]),
+    "picgpu_set_initial_ucont": (C.c_int, [_P, C.c_int, _P]),
+    "picgpu_boundary_execute": (C.c_int, [_P]),
+    "picgpu_contra2cart": (C.c_int, [_P]),
+    "picgpu_set_eulerian_fields": (C.c_int, [_P, _I64, _I64, C.c_int, _P]),
     "picgpu_mlp_param_count": (_I64, []),
     "picgpu_mlp_set_params": (C.c_int, [_P, _P]),
     "picgpu_mlp_get_params": (C.c_int, [_P, _P]),

@@ -179,6 +179,7 @@ static int finish_grid(picgpu_ctx *c) {
   c->have_grid = true;
   c->have_planes = false;
   c->have_ucat = false;
+  c->eul.have_metrics = c->eul.have_bc = false;
   c->sorted = c->keyed = c->key_ordered = false;
   c->have_pavg = c->have_faces = false;
   return PICGPU_OK;
@@ -264,6 +265,7 @@ int picgpu_destroy(picgpu_ctx *c) {
                     &c->kstart[0], &c->kstart[1], &c->kidx[0], &c->kidx[1], &c->kslot, &c->kbase, &c->planes,
                     &c->defer, &c->walk_hist};
   for (DevBuf *b : bufs) b->release();
+  eulerian_release(c);
   SwarmBufs *sws[] = {&c->sw, &c->alt};
   for (SwarmBufs *s : sws) {
     s->pos.release();
@@ -344,7 +346,7 @@ int picgpu_upload_ucat_device(picgpu_ctx *c, const double *d_ucat) {
   PG_CHECK(r);
   PG_HIP(hipStreamSynchronize(c->stream));
   c->have_ucat = true;
-  return PICGPU_OK;
+  return eulerian_take_ucat(c, d_ucat, hipMemcpyDeviceToDevice);
 }
 
 int picgpu_init_particles_device(picgpu_ctx *c, int64_t n, const double *d_pos, int64_t pid_base) {
@@ -392,7 +394,7 @@ int picgpu_upload_ucat(picgpu_ctx *c, const double *ucat) {
         }
     PG_HIP(hipMemcpy(c->ucat.ptr, h.data(), sizeof(R) * h.size(), hipMemcpyHostToDevice));
     c->have_ucat = true;
-    return PICGPU_OK;
+    return eulerian_take_ucat(c, ucat, hipMemcpyHostToDevice);
   });
 }
 
@@ -1109,7 +1111,10 @@ int picgpu_grid_field_size(picgpu_ctx *c, int which, int64_t *count) {
     case PICGPU_FIELD_IFACE_VOLFRAC:
     case PICGPU_FIELD_JFACE_VOLFRAC:
     case PICGPU_FIELD_KFACE_VOLFRAC: *count = nface(c, which - PICGPU_FIELD_IFACE_VOLFRAC); break;
-    default: set_error("unknown field %d", which); return PICGPU_E_ARG;
+    default: {
+      const double *p = nullptr;
+      return eulerian_field(c, which, &p, count);
+    }
   }
   return PICGPU_OK;
 }
@@ -1120,6 +1125,13 @@ int picgpu_download_grid_field(picgpu_ctx *c, int which, double *out) {
   PG_CHECK(need_grid(c));
   PG_HIP(hipStreamSynchronize(c->stream));
   const int64_t nx = c->IM + 1, ny = c->JM + 1;
+  if (which >= PICGPU_FIELD_UCAT && which <= PICGPU_FIELD_AJ) {
+    const double *p = nullptr;
+    int64_t count = 0;
+    PG_CHECK(eulerian_field(c, which, &p, &count));
+    PG_HIP(hipMemcpy(out, p, sizeof(double) * (size_t)count, hipMemcpyDeviceToHost));
+    return PICGPU_OK;
+  }
   if (which == PICGPU_FIELD_PARTICLE_COUNT) {
     if (!c->sorted && !c->keyed) {
       set_error("ParticleCount needs picgpu_scatter/picgpu_deposit_faces/picgpu_step first");

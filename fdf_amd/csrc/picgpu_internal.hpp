@@ -118,6 +118,19 @@ struct picgpu_ctx {
   picgpu::DevBuf ucat;    // real [ncell][3] (compact, cell-linear)
   picgpu::DevBuf corner;  // real [nnode][3]
 
+  // Eulerian field preparation (eulerian.hip); node-slot FP64 arrays
+  struct {
+    picgpu::DevBuf csi, eta, zet;  // double [nnode][3] face metric vectors
+    picgpu::DevBuf aj;             // double [nnode] inverse Jacobian
+    picgpu::DevBuf ucont, ucat;    // double [nnode][3] the reference's Ucont and Ucat
+    picgpu::DevBuf flag;           // uint64 [2]: first singular node / scratch
+    picgpu::DevBuf part;           // double [blocks][2] min/max partials
+    bool have_metrics = false;     // metrics computed for the current grid
+    bool have_bc = false;          // boundary system created
+    int orientation = 1;           // GridOrientation
+    picgpu_boundary_config bc{};
+  } eul;
+
   // swarm (double-buffered for sorting / compaction)
   int64_t n = 0;
   picgpu::SwarmBufs sw, alt;
@@ -184,6 +197,14 @@ int flush_timers(picgpu_ctx *c);
 
 // device exclusive scan of int32 (n entries -> out[0..n], out[n] = total)
 int scan_exclusive_i32(picgpu_ctx *c, const int32_t *in, int32_t *out, int64_t n);
+
+// Eulerian field preparation (eulerian.hip)
+void eulerian_release(picgpu_ctx *c);
+// device pointer and element count of an Eulerian node field (PICGPU_FIELD_UCAT..AJ)
+int eulerian_field(picgpu_ctx *c, int which, const double **d_ptr, int64_t *count);
+// keeps the Eulerian Ucat (when metrics exist) equal to a freshly uploaded
+// node-slot Ucat; kind: hipMemcpyHostToDevice or hipMemcpyDeviceToDevice
+int eulerian_take_ucat(picgpu_ctx *c, const double *ucat, hipMemcpyKind kind);
 
 // MLP
 int mlp_guess_device(picgpu_ctx *c, int64_t n, const double *d_pos, const int32_t *d_cell_or_null,

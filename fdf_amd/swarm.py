@@ -12,6 +12,16 @@ PetscErrorCode); per-particle outcomes are statuses.
     InterpolateAllFieldsToSwarm       interpolation.c:1660-1711
     ScatterAllParticleFieldsToEulerFields interpolation.c:2223-2313
     AdvanceSimulation                 the particle stages of AdvanceSimulation_TEST simulation.c:248-292
+
+Eulerian field preparation (the stage before the particle loop):
+
+    compute_metrics                   ComputeFaceMetrics + ComputeCellCenteredJacobianInverse
+                                      + CheckAndFixGridOrientation  Metric.c:195-622
+    BoundarySystem_Create             Boundaries.c:467-565
+    SetInitialInteriorField           AnalyticalSolution.c:986-1176
+    BoundarySystem_ExecuteStep        Boundaries.c:566-646
+    Contra2Cart                       setup.c:1294-1420
+    SetEulerianFields                 simulation.c:38-107
 """
 from __future__ import annotations
 
@@ -174,6 +184,41 @@ class PicGpu:
         flags = (L.STEP_RK2 if rk2 else 0) | (L.STEP_DEPOSIT_CELLS if deposit_cells else 0) | \
                 (L.STEP_DEPOSIT_FACES if deposit_faces else 0) | (L.STEP_STAGED if staged else 0)
         L.check(self.lib.picgpu_step(self.h, dt, nsteps, flags))
+
+    # ---- Eulerian field preparation ----
+    def compute_metrics(self) -> int:
+        """Csi/Eta/Zet, Aj and the orientation check; returns GridOrientation."""
+        o = C.c_int()
+        L.check(self.lib.picgpu_compute_metrics(self.h, C.byref(o)))
+        return o.value
+
+    def BoundarySystem_Create(self, config):
+        """config: a BoundaryConfig or the path of a bcs.dat-format file."""
+        from .boundary import BoundaryConfig
+
+        cfg = config if isinstance(config, BoundaryConfig) else BoundaryConfig.from_file(config)
+        b = cfg.to_c()
+        L.check(self.lib.picgpu_boundary_create(self.h, C.byref(b)))
+        self.boundary = cfg
+
+    def upload_ucont(self, ucont: np.ndarray):
+        ucont = np.ascontiguousarray(ucont, dtype=np.float64)
+        assert ucont.size == 3 * (self.IM + 1) * (self.JM + 1) * (self.KM + 1)
+        L.check(self.lib.picgpu_upload_ucont(self.h, _ptr(ucont)))
+
+    def SetInitialInteriorField(self, finit: int, uin=(0.0, 0.0, 0.0)):
+        u = np.ascontiguousarray(uin, np.float64)
+        L.check(self.lib.picgpu_set_initial_ucont(self.h, int(finit), _ptr(u)))
+
+    def BoundarySystem_ExecuteStep(self):
+        L.check(self.lib.picgpu_boundary_execute(self.h))
+
+    def Contra2Cart(self):
+        L.check(self.lib.picgpu_contra2cart(self.h))
+
+    def SetEulerianFields(self, step: int, start_step: int = 0, finit: int = 0, uin=(0.0, 0.0, 0.0)):
+        u = np.ascontiguousarray(uin, np.float64)
+        L.check(self.lib.picgpu_set_eulerian_fields(self.h, int(step), int(start_step), int(finit), _ptr(u)))
 
     def synchronize(self):
         L.check(self.lib.picgpu_synchronize(self.h))

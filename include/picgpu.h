@@ -44,6 +44,9 @@ extern "C" {
 #define PICGPU_E_HIP (-4)        /* HIP runtime error (PETSC_ERR_LIB) */
 #define PICGPU_E_NOMEM (-5)      /* device allocation failed (PETSC_ERR_MEM) */
 #define PICGPU_E_UNSUPPORTED (-6)/* (PETSC_ERR_SUP) */
+#define PICGPU_E_SINGULAR (-7)   /* near-zero Jacobian (Metric.c:555-557) or metric determinant
+                                    (setup.c:1366-1368) (PETSC_ERR_FLOP_COUNT) */
+#define PICGPU_E_GRID (-8)       /* mixed Jacobian signs (Metric.c:217-220) (PETSC_ERR_USER) */
 
 /* ---- particle status (common.h:75-124) ---- */
 #define PICGPU_NEEDS_LOCATION 0
@@ -66,6 +69,13 @@ extern "C" {
 #define PICGPU_FIELD_IFACE_VOLFRAC 6  /* [KM][JM][IM+1]: count * V_p / V_dual */
 #define PICGPU_FIELD_JFACE_VOLFRAC 7
 #define PICGPU_FIELD_KFACE_VOLFRAC 8
+/* Eulerian fields (picgpu_compute_metrics first), node slots [(KM+1)(JM+1)(IM+1)] */
+#define PICGPU_FIELD_UCAT 9           /* Cartesian velocity Ucat x3 (ghost planes included) */
+#define PICGPU_FIELD_UCONT 10         /* contravariant velocity Ucont x3 */
+#define PICGPU_FIELD_CSI 11           /* i-face metric vectors x3 */
+#define PICGPU_FIELD_ETA 12           /* j-face metric vectors x3 */
+#define PICGPU_FIELD_ZET 13           /* k-face metric vectors x3 */
+#define PICGPU_FIELD_AJ 14            /* inverse Jacobian */
 
 /* ---- picgpu_step flags ---- */
 #define PICGPU_STEP_RK2 0x1           /* Heun predictor-corrector instead of forward Euler */
@@ -169,6 +179,73 @@ int picgpu_walk_stats_enable(picgpu_ctx *ctx, int on);
 int picgpu_timing_enable(picgpu_ctx *ctx, int on);
 int picgpu_timing_query(picgpu_ctx *ctx, const char *kernel, double *total_ms, int64_t *launches);
 int picgpu_timing_reset(picgpu_ctx *ctx);
+
+/* ---- Eulerian field preparation: the stage before the particle loop every step ----
+ * Reference entry points each call replaces (one MPI rank):
+ *   picgpu_compute_metrics     <- ComputeFaceMetrics, ComputeCellCenteredJacobianInverse,
+ *                                 CheckAndFixGridOrientation   Metric.c:195-622 (setup.c:279-281)
+ *   picgpu_parse_boundary_file <- ParseAllBoundaryConditions   io.c:2515-2680 (+ handler parameters,
+ *                                 BC_Handlers.c:238-256, 647-664)
+ *   picgpu_boundary_create     <- BoundarySystem_Create         Boundaries.c:467-565
+ *   picgpu_set_initial_ucont   <- SetInitialInteriorField("Ucont") AnalyticalSolution.c:986-1176
+ *   picgpu_boundary_execute    <- BoundarySystem_ExecuteStep    Boundaries.c:566-646
+ *   picgpu_contra2cart         <- Contra2Cart                   setup.c:1294-1420
+ *   picgpu_set_eulerian_fields <- SetEulerianFields             simulation.c:38-107
+ * All FP64 in both precision modes; the particle stages read the Ucat these
+ * calls produce (or the one picgpu_upload_ucat* gave, whichever came last). */
+#define PICGPU_FACE_NEG_X 0 /* BCFace, common.h:202-206 */
+#define PICGPU_FACE_POS_X 1
+#define PICGPU_FACE_NEG_Y 2
+#define PICGPU_FACE_POS_Y 3
+#define PICGPU_FACE_NEG_Z 4
+#define PICGPU_FACE_POS_Z 5
+#define PICGPU_BCTYPE_UNDEFINED 0 /* BCType, common.h:209-212 */
+#define PICGPU_BCTYPE_NOGRAD 1
+#define PICGPU_BCTYPE_WALL 2
+#define PICGPU_BCTYPE_SYMMETRY 3
+#define PICGPU_BCTYPE_INLET 4
+#define PICGPU_BCTYPE_OUTLET 5
+#define PICGPU_BC_UNDEFINED 0 /* BCHandlerType, common.h:215-222 */
+#define PICGPU_BC_NOGRAD_COPY_GHOST 1
+#define PICGPU_BC_WALL_NOSLIP 2
+#define PICGPU_BC_WALL_MOVING 3
+#define PICGPU_BC_SYMMETRY_PLANE 4
+#define PICGPU_BC_INLET_PARABOLIC 5
+#define PICGPU_BC_INLET_CONSTANT_VELOCITY 6
+#define PICGPU_BC_OUTLET_CONSERVATION 9
+
+typedef struct {
+  int type[6];              /* PICGPU_BCTYPE_* per face (PICGPU_FACE_* order) */
+  int handler[6];           /* PICGPU_BC_* per face */
+  double velocity[6][3];    /* constant_velocity parameters vx vy vz (default 0) */
+  double vmax[6];           /* parabolic parameter vmax (default 1) */
+} picgpu_boundary_config;
+
+/* every face WALL / noslip, parameters at their defaults (io.c:2531-2537) */
+void picgpu_default_boundary_config(picgpu_boundary_config *bc);
+/* "<face> <type> <handler> [key=value ...]" lines, '#' comments; faces -Xi +Xi
+ * -Eta +Eta -Zeta +Zeta, types WALL SYMMETRY INLET OUTLET NOGRAD, handlers
+ * noslip constant_velocity conservation allcopy parabolic (case-insensitive).
+ * Unknown names, an invalid type/handler pair or an unreadable file: PICGPU_E_ARG. */
+int picgpu_parse_boundary_file(const char *path, picgpu_boundary_config *bc);
+/* Face metrics Csi/Eta/Zet, inverse Jacobian Aj, orientation check (a
+ * left-handed grid is flipped; *orientation, may be NULL, receives the
+ * reference's GridOrientation).  Sets Ucat and Ucont to zero (setup.c:250-268). */
+int picgpu_compute_metrics(picgpu_ctx *ctx, int *orientation);
+/* Stores the configuration and runs the inlet handlers' initial application.
+ * Handlers other than allcopy, noslip, constant_velocity and parabolic:
+ * PICGPU_E_UNSUPPORTED (Boundaries.c:402-466). */
+int picgpu_boundary_create(picgpu_ctx *ctx, const picgpu_boundary_config *bc);
+/* Ucont in the node-slot layout (e.g. read from a restart file) */
+int picgpu_upload_ucont(picgpu_ctx *ctx, const double *ucont);
+int picgpu_upload_ucont_device(picgpu_ctx *ctx, const double *d_ucont);
+/* finit 0 zero, 1 constant, 2 parabolic profile (the -finit option); uin = -ucont_x/y/z */
+int picgpu_set_initial_ucont(picgpu_ctx *ctx, int finit, const double uin[3]);
+int picgpu_boundary_execute(picgpu_ctx *ctx);
+int picgpu_contra2cart(picgpu_ctx *ctx);
+/* step == start_step > 0 (restart): nothing (the caller uploaded Ucat and Ucont);
+ * otherwise [step 0: initial Ucont], boundaries, Contra2Cart, boundaries. */
+int picgpu_set_eulerian_fields(picgpu_ctx *ctx, int64_t step, int64_t start_step, int finit, const double uin[3]);
 
 /* ---- MLP initial guess (3 -> 64 -> 64 -> 3, GELU, BF16 weights, FP32 accumulation) ---- */
 #define PICGPU_MLP_HIDDEN 64

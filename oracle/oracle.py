@@ -32,6 +32,23 @@ class _Grid(C.Structure):
     _fields_ = [("IM", C.c_int64), ("JM", C.c_int64), ("KM", C.c_int64), ("coor", C.c_void_p)]
 
 
+class _BC(C.Structure):
+    _fields_ = [("type", C.c_int * 6), ("handler", C.c_int * 6), ("vel", (C.c_double * 3) * 6),
+                ("vmax", C.c_double * 6)]
+
+
+def _bc(cfg) -> _BC:
+    """cfg: any object with type[6], handler[6], vel[6][3], vmax[6]."""
+    b = _BC()
+    for f in range(6):
+        b.type[f] = int(cfg.type[f])
+        b.handler[f] = int(cfg.handler[f])
+        b.vmax[f] = float(cfg.vmax[f])
+        for c in range(3):
+            b.vel[f][c] = float(cfg.vel[f][c])
+    return b
+
+
 def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
@@ -64,6 +81,15 @@ class Oracle:
             "ref_deposit_faces": (None, [I64, I64, I64, I64, P, P, P, P, P, P, P]),
             "ref_step_euler": (I64, [C.POINTER(_Ctx), I64, P, P, P, P, P, P, P, D, P]),
             "ref_step_rk2": (I64, [C.POINTER(_Ctx), I64, P, P, P, P, P, P, P, D, P]),
+            "ref_face_metrics": (None, [I64, I64, I64, P, P, P, P]),
+            "ref_jacobian_inverse": (I, [I64, I64, I64, P, P]),
+            "ref_grid_orientation": (I, [I64, P, P, P, P, P]),
+            "ref_inlet_face": (I, [C.POINTER(_BC)]),
+            "ref_initial_ucont": (None, [I64, I64, I64, P, P, P, I, P, I, I, P]),
+            "ref_bc_initialize": (I, [I64, I64, I64, C.POINTER(_BC), P, P, P, P, P]),
+            "ref_bc_execute": (I, [I64, I64, I64, C.POINTER(_BC), P, P, P, P, P]),
+            "ref_contra2cart": (I, [I64, I64, I64, P, P, P, P, P]),
+            "ref_set_eulerian_fields": (I, [I64, I64, I64, C.POINTER(_BC), P, P, P, I64, I, P, I, P, P]),
             "ref_mlp_param_count": (I64, [I]),
             "ref_mlp_forward": (None, [I, P, I64, P, P, I]),
             "ref_mlp_loss_grad": (D, [I, P, I64, P, P, P, I]),
@@ -179,6 +205,58 @@ class Oracle:
         for k in s:
             s[k] = s[k][:m]
         return removed.value
+
+    # ---- Eulerian field preparation (node-slot arrays [(KM+1)(JM+1)(IM+1)](x3)) ----
+    def metrics(self, IM, JM, KM, coor):
+        """Face metrics, Jacobian inverse and the orientation check (setup.c:279-281).
+        Returns (csi, eta, zet, aj, orientation); raises on the reference's errors."""
+        coor = np.ascontiguousarray(coor, np.float64)
+        nn = (IM + 1) * (JM + 1) * (KM + 1)
+        csi, eta, zet = (np.zeros((nn, 3)) for _ in range(3))
+        aj = np.zeros(nn)
+        self.lib.ref_face_metrics(IM, JM, KM, _p(coor), _p(csi), _p(eta), _p(zet))
+        r = self.lib.ref_jacobian_inverse(IM, JM, KM, _p(coor), _p(aj))
+        if r != 0:
+            raise RuntimeError(f"oracle jacobian error {r}")
+        o = C.c_int()
+        r = self.lib.ref_grid_orientation(nn, _p(csi), _p(eta), _p(zet), _p(aj), C.byref(o))
+        if r != 0:
+            raise RuntimeError(f"oracle orientation error {r}")
+        return csi, eta, zet, aj, o.value
+
+    def inlet_face(self, cfg):
+        b = _bc(cfg)
+        return self.lib.ref_inlet_face(C.byref(b))
+
+    def initial_ucont(self, IM, JM, KM, csi, eta, zet, finit, uin, inlet_face, orientation, ucont):
+        uin = np.ascontiguousarray(uin, np.float64)
+        self.lib.ref_initial_ucont(IM, JM, KM, _p(csi), _p(eta), _p(zet), int(finit), _p(uin), int(inlet_face),
+                                   int(orientation), _p(ucont))
+
+    def bc_initialize(self, IM, JM, KM, cfg, csi, eta, zet, ucat, ucont):
+        b = _bc(cfg)
+        r = self.lib.ref_bc_initialize(IM, JM, KM, C.byref(b), _p(csi), _p(eta), _p(zet), _p(ucat), _p(ucont))
+        if r != 0:
+            raise RuntimeError(f"oracle bc error {r}")
+
+    def bc_execute(self, IM, JM, KM, cfg, csi, eta, zet, ucat, ucont):
+        b = _bc(cfg)
+        r = self.lib.ref_bc_execute(IM, JM, KM, C.byref(b), _p(csi), _p(eta), _p(zet), _p(ucat), _p(ucont))
+        if r != 0:
+            raise RuntimeError(f"oracle bc error {r}")
+
+    def contra2cart(self, IM, JM, KM, ucont, csi, eta, zet, ucat):
+        r = self.lib.ref_contra2cart(IM, JM, KM, _p(ucont), _p(csi), _p(eta), _p(zet), _p(ucat))
+        if r != 0:
+            raise RuntimeError(f"oracle contra2cart error {r}")
+
+    def set_eulerian_fields(self, IM, JM, KM, cfg, csi, eta, zet, step, finit, uin, orientation, ucat, ucont):
+        b = _bc(cfg)
+        uin = np.ascontiguousarray(uin, np.float64)
+        r = self.lib.ref_set_eulerian_fields(IM, JM, KM, C.byref(b), _p(csi), _p(eta), _p(zet), int(step),
+                                             int(finit), _p(uin), int(orientation), _p(ucat), _p(ucont))
+        if r != 0:
+            raise RuntimeError(f"oracle eulerian error {r}")
 
     # ---- MLP (spec) ----
     def mlp_forward(self, params, x, round_bf16=True, H=64):

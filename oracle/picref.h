@@ -128,6 +128,38 @@ int64_t ref_step_rk2(const ref_ctx_t *c, int64_t n, double *pos, double *vel, in
                      double *w, int64_t *status, int64_t *pid, double *P, double dt,
                      int64_t *removed);
 
+/* ---- Eulerian field preparation (eulerref.c) ----
+ * Node-slot arrays [(KM+1)(JM+1)(IM+1)], vectors x3, i fastest. */
+enum { REF_E_SINGULAR = -7, REF_E_GRID = -8 };
+/* BCType (common.h:209-212) and BCHandlerType (common.h:215-222) values */
+enum { REF_BCTYPE_WALL = 2, REF_BCTYPE_INLET = 4 };
+enum { REF_BC_NOGRAD_COPY_GHOST = 1, REF_BC_WALL_NOSLIP = 2, REF_BC_INLET_PARABOLIC = 5,
+       REF_BC_INLET_CONSTANT_VELOCITY = 6 };
+typedef struct {
+  int type[6];          /* BCType per face, faces -Xi +Xi -Eta +Eta -Zeta +Zeta */
+  int handler[6];       /* BCHandlerType per face */
+  double vel[6][3];     /* constant_velocity: vx vy vz */
+  double vmax[6];       /* parabolic: vmax */
+} ref_bc_t;
+void ref_face_metrics(int64_t IM, int64_t JM, int64_t KM, const double *coor, double *csi, double *eta,
+                      double *zet);                                               /* Metric.c:286-490 */
+int ref_jacobian_inverse(int64_t IM, int64_t JM, int64_t KM, const double *coor, double *aj); /* Metric.c:491-622 */
+int ref_grid_orientation(int64_t nnode, double *csi, double *eta, double *zet, double *aj,
+                         int *orientation);                                       /* Metric.c:195-284 */
+int ref_inlet_face(const ref_bc_t *bc);                                           /* io.c:2664-2672 */
+void ref_initial_ucont(int64_t IM, int64_t JM, int64_t KM, const double *csi, const double *eta,
+                       const double *zet, int finit, const double uin[3], int inlet_face, int orientation,
+                       double *ucont);                                            /* AnalyticalSolution.c:986-1176 */
+int ref_bc_initialize(int64_t IM, int64_t JM, int64_t KM, const ref_bc_t *bc, const double *csi,
+                      const double *eta, const double *zet, double *ucat, double *ucont); /* Boundaries.c:467-565 */
+int ref_bc_execute(int64_t IM, int64_t JM, int64_t KM, const ref_bc_t *bc, const double *csi,
+                   const double *eta, const double *zet, double *ucat, double *ucont); /* Boundaries.c:566-646 */
+int ref_contra2cart(int64_t IM, int64_t JM, int64_t KM, const double *ucont, const double *csi, const double *eta,
+                    const double *zet, double *ucat);                             /* setup.c:1294-1420 */
+int ref_set_eulerian_fields(int64_t IM, int64_t JM, int64_t KM, const ref_bc_t *bc, const double *csi,
+                            const double *eta, const double *zet, int64_t step, int finit, const double uin[3],
+                            int orientation, double *ucat, double *ucont);        /* simulation.c:38-107 */
+
 /* ---- SPEC: MLP initial guess, 3 -> H -> H -> 3, GELU(erf) ---- */
 /* params layout: W1[H][3], b1[H], W2[H][H], b2[H], W3[3][H], b3[3] (row = output) */
 int64_t ref_mlp_param_count(int H);
