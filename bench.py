@@ -148,6 +148,20 @@ def run_variant(name: str, d: dict, steps: int, warmup: int, mlp: dict | None):
     g = PicGpu(precision=precision, use_mlp_guess=False)
     g.upload_grid_device(IM, JM, KM, d["coor"].data_ptr())
     g.upload_ucat_device(d["ucat"].data_ptr())
+    fields = "ucont" in d
+    if fields:
+        # Ucat comes from the field preparation: metrics, zero-gradient faces,
+        # the synthetic field's Ucont, then one preparation (boundaries,
+        # Contra2Cart, boundaries); every step repeats the preparation
+        from fdf_amd.boundary import BoundaryConfig
+
+        g.compute_metrics()
+        bc = BoundaryConfig()
+        for f in range(6):
+            bc.set_face(f, "NOGRAD", "allcopy")
+        g.BoundarySystem_Create(bc)
+        g.upload_ucont_device(d["ucont"].data_ptr())
+        g.SetEulerianFields(1, 0)
     n0 = int(d["pos_dev"].shape[0])
     g.init_particles_device(n0, d["pos_dev"].data_ptr())
     torch.cuda.synchronize()
@@ -163,7 +177,8 @@ def run_variant(name: str, d: dict, steps: int, warmup: int, mlp: dict | None):
     res["initial_located"], res["initial_lost"] = st0["located"], st0["lost"]
     res["initial_walk_iterations_per_particle"] = st0["walk_iterations"] / max(1, n0)
     log(f"{name}: initial locate {res['initial_locate_s']:.2f}s located={st0['located']} lost={st0['lost']}")
-    flags = dict(rk2=d["rk2"], deposit_cells=("cells" in d["deposit"]), deposit_faces=("faces" in d["deposit"]))
+    flags = dict(rk2=d["rk2"], deposit_cells=("cells" in d["deposit"]), deposit_faces=("faces" in d["deposit"]),
+                 fields=fields)
     for _ in range(warmup):
         g.AdvanceSimulation(d["dt"], 1, **flags)
     torch.cuda.synchronize()
@@ -189,11 +204,23 @@ def run_variant(name: str, d: dict, steps: int, warmup: int, mlp: dict | None):
               "key_scatter", "key_reindex", "step_deferred", "cell_planes", "bucket_pid_order", "iota", "gather_swarm", "locate_fp64", "locate_fp32",
               "interp", "center_to_corner", "rk2_predict", "rk2_correct", "keep_flags", "compact", "cell_histogram",
               "sort_scatter", "scan_tile", "scan_add", "scatter_P", "deposit_faces", "reset_advect", "compact_tail",
-              "keep_unlocated"):
+              "keep_unlocated", "contra2cart", "bc_face", "flag_fill"):
         ms, cnt = g.timing_query(k)
         if cnt:
             kernels[k] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt}
     res["kernels"] = kernels
+    if fields and "contra2cart" in kernels:
+        # Contra2Cart per interior cell: Ucont, Csi, Eta, Zet read (4 x 24 B),
+        # Ucat written (24 B) and its cell-linear copy (3 reals)
+        rs = 8 if precision == "fp64" else 4
+        ncell_in = (IM - 1) * (JM - 1) * (KM - 1)
+        byts = ncell_in * (4 * 24 + 24 + 3 * rs)
+        avg = kernels["contra2cart"]["avg_ms"]
+        res["fields"] = {"contra2cart_avg_ms": avg, "algorithmic_bytes": byts,
+                         "achieved_GBs": byts / (avg * 1e-3) / 1e9,
+                         "frac": byts / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "cells_per_s": ncell_in / (avg * 1e-3),
+                         "bc_face_ms_per_step": kernels.get("bc_face", {}).get("total_ms", 0.0) / max(1, steps)}
     g.timing(False)
     # walk iterations per particle-step, from one more (untimed) step with statistics on
     g.walk_stats(True)
@@ -343,6 +370,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--error-sample", type=int, default=1 << 22,
                     help="particles of the FP32-vs-FP64 one-step error sample (0: skip)")
+    ap.add_argument("--fields", action="store_true",
+                    help="every step first prepares Ucat from Ucont (boundaries, Contra2Cart, boundaries), "
+                         "as the reference's SetEulerianFields does each step")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -362,6 +392,11 @@ def main():
     t0 = time.perf_counter()
     d = make_inputs(args.config, args.scale, dev)
     d["workload_name"] = f"config{args.config} scale={args.scale:g}"
+    if args.fields:
+        from fdf_amd import workloads as W
+
+        d["ucont"] = W.contravariant_from_cartesian(d["coor"], d["ucat"], d["IM"], d["JM"], d["KM"]).contiguous()
+        d["workload_name"] += " fields"
     torch.cuda.synchronize()
     calibration_copy()
     log(f"inputs: config {args.config} {d['IM']}x{d['JM']}x{d['KM']} cells, {d['n']} particles, "
@@ -417,7 +452,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"config{args.config}", "grid_cells": [d["IM"], d["JM"], d["KM"]],
                    "particles": d["n"], "scale": args.scale, "advection": "RK2" if d["rk2"] else "Euler",
-                   "deposition": d["deposit"], "dt": d["dt"], "variant": head["variant"]},
+                   "deposition": d["deposit"], "dt": d["dt"], "variant": head["variant"],
+                   "eulerian_preparation_each_step": bool(args.fields)},
         "roofline": {k: head["roofline"][k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
         "roofline_kernel": head["roofline"]["kernel"],
         "cpu_baseline": cpu,

@@ -42,6 +42,7 @@ BC_INLET_PARABOLIC, BC_INLET_CONSTANT_VELOCITY, BC_OUTLET_CONSERVATION = 5, 6, 9
 STEP_RK2 = 0x1
 STEP_DEPOSIT_CELLS = 0x2
 STEP_DEPOSIT_FACES = 0x4
+STEP_FIELDS = 0x8
 STEP_STAGED = 0x100
 
 

@@ -23,8 +23,8 @@ using namespace picgpu;
 namespace {
 
 constexpr int kEBlock = 256;
-// Contra2Cart tile: 64 i x 4 j threads marching over kMarch k planes
-constexpr int kTx = 64, kTy = 4, kMarch = 16;
+// Contra2Cart tile: 64 i x 8 j threads marching over kMarch k planes
+constexpr int kTx = 64, kTy = 8, kMarch = 32;
 
 inline int64_t eblocks(int64_t n) { return (n + kEBlock - 1) / kEBlock; }
 
@@ -331,52 +331,136 @@ __global__ __launch_bounds__(kEBlock) void k_bc_face(NodeDims d, FaceBC bc, cons
   mirror_store(mirror, IM, JM, KM, i, j, k, out);
 }
 
+// The eight nodes of the box (i-1..i, j-1..j, k-1..k): a[dk][dj][di], index 1
+// for the node's own coordinate, 0 for the one below.  The face vectors of
+// cell slot (i,j,k) that Contra2Cart averages are all functions of them, by
+// the expressions of csi_at / eta_at / zet_at.
+struct Box {
+  V3 a[2][2][2];
+};
+__device__ __forceinline__ V3 box_csi(const Box &b, int di) {
+  return cross(half_diff(b.a[1][1][di], b.a[0][1][di], b.a[1][0][di], b.a[0][0][di]),
+               half_diff(b.a[1][0][di], b.a[1][1][di], b.a[0][0][di], b.a[0][1][di]));
+}
+__device__ __forceinline__ V3 box_eta(const Box &b, int dj) {
+  const V3 dxi = half_diff(b.a[1][dj][1], b.a[0][dj][1], b.a[1][dj][0], b.a[0][dj][0]);
+  const V3 dzeta = half_diff(b.a[1][dj][1], b.a[1][dj][0], b.a[0][dj][1], b.a[0][dj][0]);
+  return cross(dzeta, dxi);
+}
+__device__ __forceinline__ V3 box_zet(const Box &b, int dk) {
+  const V3 dxi = half_diff(b.a[dk][1][1], b.a[dk][0][1], b.a[dk][1][0], b.a[dk][0][0]);
+  const V3 deta = half_diff(b.a[dk][1][1], b.a[dk][1][0], b.a[dk][0][1], b.a[dk][0][0]);
+  return cross(dxi, deta);
+}
+__device__ __forceinline__ V3 scale3(V3 v, double s) { return {v.x * s, v.y * s, v.z * s}; }
+
 // Contra2Cart (setup.c:1334-1406) for interior cells 1 <= i <= mx-2 (likewise
-// j, k).  A block is a 64 x 4 tile of (i, j) marching over kMarch k planes;
-// the k-1 face values (zet, Ucont.z) carry over in registers, the i-1 and
-// j-1 neighbours come from the tile's own loads through L1.  Cramer's rule
-// in the reference's expression order; |det| < 1e-18 reports the node.
+// j, k).  A block is a kTx x kTy tile of (i, j) marching over kMarch k planes.
+// Instead of reading the stored Csi/Eta/Zet (72 B per cell) it stages each
+// plane of node coordinates ((kTx+1) x (kTy+1) nodes, 24 B per cell plus the halo) in
+// LDS and recomputes the six face vectors of each cell with the expressions
+// of k_face_metrics, times `flip` (1, or -1 after a left-handed grid was
+// flipped), so they are the stored values bit for bit: for interior cells
+// none of them is a boundary copy.  zet and Ucont.z of plane k-1 carry over
+// in registers.  Cramer's rule in the reference's expression order;
+// |det| < 1e-18 reports the node.
 template <typename R>
-__global__ __launch_bounds__(kTx *kTy) void k_contra2cart(NodeDims d, const double *__restrict__ ucont,
-                                                          const double *__restrict__ csi,
-                                                          const double *__restrict__ eta,
-                                                          const double *__restrict__ zet, double *__restrict__ ucat,
+__global__ __launch_bounds__(kTx *kTy) void k_contra2cart(NodeDims d, const double *__restrict__ X, double flip,
+                                                          const double *__restrict__ ucont, double *__restrict__ ucat,
                                                           R *__restrict__ mirror, int IM, int JM, int KM,
                                                           unsigned long long *__restrict__ bad) {
-  const int i = 1 + (int)blockIdx.x * kTx + (int)threadIdx.x;
-  const int j = 1 + (int)blockIdx.y * kTy + (int)threadIdx.y;
+  constexpr int RowD = (kTx + 1) * 3;        // doubles per staged node row
+  constexpr int PlaneD = (kTy + 1) * RowD;   // doubles per staged plane
+  __shared__ double tile[2][PlaneD];
+  const int tx = (int)threadIdx.x, ty = (int)threadIdx.y, t = ty * kTx + tx;
+  const int i0 = 1 + (int)blockIdx.x * kTx, j0 = 1 + (int)blockIdx.y * kTy;
+  const int i = i0 + tx, j = j0 + ty;
   const int k0 = 1 + (int)blockIdx.z * kMarch;
-  if (i > d.mx - 2 || j > d.my - 2) return;
   const int k1 = min(k0 + kMarch, d.mz - 1);
-  const int64_t plane = (int64_t)d.mx * d.my;
-  int64_t n = d.at(i, j, k0 - 1);
-  V3 zprev = ld3(zet, n);
-  double wprev = ucont[3 * n + 2];
-  for (int k = k0; k < k1; ++k) {
-    n += plane;
-    const V3 cm = ld3(csi, n - 1), c0 = ld3(csi, n);
-    const V3 em = ld3(eta, n - d.mx), e0 = ld3(eta, n);
-    const V3 z0 = ld3(zet, n);
-    const double um = ucont[3 * (n - 1)], vm = ucont[3 * (n - d.mx) + 1];
-    const V3 u0 = ld3(ucont, n);
-    const double m00 = 0.5 * (cm.x + c0.x), m01 = 0.5 * (cm.y + c0.y), m02 = 0.5 * (cm.z + c0.z);
-    const double m10 = 0.5 * (em.x + e0.x), m11 = 0.5 * (em.y + e0.y), m12 = 0.5 * (em.z + e0.z);
-    const double m20 = 0.5 * (zprev.x + z0.x), m21 = 0.5 * (zprev.y + z0.y), m22 = 0.5 * (zprev.z + z0.z);
-    const double q0 = 0.5 * (um + u0.x), q1 = 0.5 * (vm + u0.y), q2 = 0.5 * (wprev + u0.z);
-    zprev = z0;
-    wprev = u0.z;
-    const double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
-    if (fabs(det) < 1.0e-18) {
-      atomicMin(bad, (unsigned long long)n);
-      continue;
+  const bool live = i <= d.mx - 2 && j <= d.my - 2;
+  const int rmax = 3 * (d.mx - i0 + 1);  // staged doubles of a row that lie inside the grid
+  // each thread stages kPer doubles of a plane: fetched into registers one
+  // plane ahead (the loads fly while the current plane is computed), then
+  // committed to LDS
+  constexpr int kNT = kTx * kTy, kPer = (PlaneD + kNT - 1) / kNT;
+  double pre[kPer];
+  auto fetch = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int f = t + q * kNT;
+      const int jj = f / RowD, r = f - jj * RowD;
+      const int jn = j0 - 1 + jj;
+      pre[q] = (f < PlaneD && r < rmax && jn <= d.my - 1) ? X[3 * d.at(i0 - 1, jn, k) + r] : 0.0;
     }
-    const double rdet = 1.0 / det;
-    const double d0 = q0 * (m11 * m22 - m12 * m21) - q1 * (m01 * m22 - m02 * m21) + q2 * (m01 * m12 - m02 * m11);
-    const double d1 = -q0 * (m10 * m22 - m12 * m20) + q1 * (m00 * m22 - m02 * m20) - q2 * (m00 * m12 - m02 * m10);
-    const double d2 = q0 * (m10 * m21 - m11 * m20) - q1 * (m00 * m21 - m01 * m20) + q2 * (m00 * m11 - m01 * m10);
-    const V3 out = {d0 * rdet, d1 * rdet, d2 * rdet};
-    st3(ucat, n, out);
-    mirror_store(mirror, IM, JM, KM, i, j, k, out);
+  };
+  auto commit = [&](double *dst) {
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int f = t + q * kNT;
+      if (f < PlaneD) dst[f] = pre[q];
+    }
+  };
+  auto node = [&](const double *src, int dj, int di) -> V3 {
+    const double *p = src + (ty + dj) * RowD + 3 * (tx + di);
+    return {p[0], p[1], p[2]};
+  };
+  fetch(k0 - 1);
+  commit(tile[(k0 - 1) & 1]);
+  if (k0 < k1) fetch(k0);
+  __syncthreads();
+  Box b;
+  V3 zprev = {0.0, 0.0, 0.0};
+  double wprev = 0.0;
+  const int64_t plane = (int64_t)d.mx * d.my;
+  int64_t n = live ? d.at(i, j, k0 - 1) : 0;
+  if (live) {
+    const double *lo = tile[(k0 - 1) & 1];
+#pragma unroll
+    for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+      for (int di = 0; di < 2; ++di) b.a[1][dj][di] = node(lo, dj, di);
+    zprev = scale3(box_zet(b, 1), flip);
+    wprev = ucont[3 * n + 2];
+  }
+  for (int k = k0; k < k1; ++k) {
+    const double *hi = tile[k & 1];
+    commit(tile[k & 1]);  // overwrites plane k-2, last read before the previous barrier
+    __syncthreads();
+    if (k + 1 < k1) fetch(k + 1);
+    if (live) {
+      n += plane;
+#pragma unroll
+      for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+        for (int di = 0; di < 2; ++di) {
+          b.a[0][dj][di] = b.a[1][dj][di];
+          b.a[1][dj][di] = node(hi, dj, di);
+        }
+      const V3 cm = scale3(box_csi(b, 0), flip), c0 = scale3(box_csi(b, 1), flip);
+      const V3 em = scale3(box_eta(b, 0), flip), e0 = scale3(box_eta(b, 1), flip);
+      const V3 z0 = scale3(box_zet(b, 1), flip);
+      const double um = ucont[3 * (n - 1)], vm = ucont[3 * (n - d.mx) + 1];
+      const V3 u0 = ld3(ucont, n);
+      const double m00 = 0.5 * (cm.x + c0.x), m01 = 0.5 * (cm.y + c0.y), m02 = 0.5 * (cm.z + c0.z);
+      const double m10 = 0.5 * (em.x + e0.x), m11 = 0.5 * (em.y + e0.y), m12 = 0.5 * (em.z + e0.z);
+      const double m20 = 0.5 * (zprev.x + z0.x), m21 = 0.5 * (zprev.y + z0.y), m22 = 0.5 * (zprev.z + z0.z);
+      const double q0 = 0.5 * (um + u0.x), q1 = 0.5 * (vm + u0.y), q2 = 0.5 * (wprev + u0.z);
+      zprev = z0;
+      wprev = u0.z;
+      const double det =
+          m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
+      if (fabs(det) < 1.0e-18) {
+        atomicMin(bad, (unsigned long long)n);
+      } else {
+        const double rdet = 1.0 / det;
+        const double d0 = q0 * (m11 * m22 - m12 * m21) - q1 * (m01 * m22 - m02 * m21) + q2 * (m01 * m12 - m02 * m11);
+        const double d1 = -q0 * (m10 * m22 - m12 * m20) + q1 * (m00 * m22 - m02 * m20) - q2 * (m00 * m12 - m02 * m10);
+        const double d2 = q0 * (m10 * m21 - m11 * m20) - q1 * (m00 * m21 - m01 * m20) + q2 * (m00 * m11 - m01 * m10);
+        const V3 out = {d0 * rdet, d1 * rdet, d2 * rdet};
+        st3(ucat, n, out);
+        mirror_store(mirror, IM, JM, KM, i, j, k, out);
+      }
+    }
   }
 }
 
@@ -453,23 +537,26 @@ int run_boundaries(picgpu_ctx *c, bool inlets_only) {
   return PICGPU_OK;
 }
 
-int run_contra2cart(picgpu_ctx *c) {
+int launch_contra2cart(picgpu_ctx *c) {
   const NodeDims d = node_dims(c);
   if (d.mx < 3 || d.my < 3 || d.mz < 3) return PICGPU_OK;  // no interior cell
-  PG_CHECK(flag_reset(c));
   const dim3 grid((unsigned)((d.mx - 2 + kTx - 1) / kTx), (unsigned)((d.my - 2 + kTy - 1) / kTy),
                   (unsigned)((d.mz - 2 + kMarch - 1) / kMarch));
   const dim3 block(kTx, kTy, 1);
   auto *bad = c->eul.flag.as<unsigned long long>();
+  const double flip = c->eul.flipped ? -1.0 : 1.0;
   if (c->opt.precision == PICGPU_FP32) {
-    PG_LAUNCH(c, "contra2cart", grid, block, k_contra2cart<float>, d, c->eul.ucont.as<double>(),
-              c->eul.csi.as<double>(), c->eul.eta.as<double>(), c->eul.zet.as<double>(), c->eul.ucat.as<double>(),
-              c->ucat.as<float>(), c->IM, c->JM, c->KM, bad);
+    PG_LAUNCH(c, "contra2cart", grid, block, k_contra2cart<float>, d, c->coor.as<double>(), flip,
+              c->eul.ucont.as<double>(), c->eul.ucat.as<double>(), c->ucat.as<float>(), c->IM, c->JM, c->KM, bad);
   } else {
-    PG_LAUNCH(c, "contra2cart", grid, block, k_contra2cart<double>, d, c->eul.ucont.as<double>(),
-              c->eul.csi.as<double>(), c->eul.eta.as<double>(), c->eul.zet.as<double>(), c->eul.ucat.as<double>(),
-              c->ucat.as<double>(), c->IM, c->JM, c->KM, bad);
+    PG_LAUNCH(c, "contra2cart", grid, block, k_contra2cart<double>, d, c->coor.as<double>(), flip,
+              c->eul.ucont.as<double>(), c->eul.ucat.as<double>(), c->ucat.as<double>(), c->IM, c->JM, c->KM,
+              bad);
   }
+  return PICGPU_OK;
+}
+
+int check_contra2cart(picgpu_ctx *c) {
   unsigned long long first = 0;
   PG_CHECK(flag_read(c, &first));
   if (first != ~0ull) {
@@ -479,6 +566,12 @@ int run_contra2cart(picgpu_ctx *c) {
     return PICGPU_E_SINGULAR;
   }
   return PICGPU_OK;
+}
+
+int run_contra2cart(picgpu_ctx *c) {
+  PG_CHECK(flag_reset(c));
+  PG_CHECK(launch_contra2cart(c));
+  return check_contra2cart(c);
 }
 
 int run_initial_ucont(picgpu_ctx *c, int finit, const double *uin) {
@@ -555,6 +648,19 @@ int eulerian_field(picgpu_ctx *c, int which, const double **d_ptr, int64_t *coun
   }
   return PICGPU_OK;
 }
+
+int eulerian_fields_begin(picgpu_ctx *c) {
+  PG_CHECK(need_bc(c));
+  return flag_reset(c);
+}
+
+int eulerian_fields_async(picgpu_ctx *c) {
+  PG_CHECK(run_boundaries(c, false));
+  PG_CHECK(launch_contra2cart(c));
+  return run_boundaries(c, false);
+}
+
+int eulerian_fields_end(picgpu_ctx *c) { return check_contra2cart(c); }
 
 int eulerian_take_ucat(picgpu_ctx *c, const double *ucat, hipMemcpyKind kind) {
   if (!c->eul.have_metrics) return PICGPU_OK;
@@ -707,8 +813,10 @@ int picgpu_compute_metrics(picgpu_ctx *c, int *orientation) {
     return PICGPU_E_GRID;
   }
   int o = 1;
+  e.flipped = false;
   if (hi < 0.0) {
     o = -1;
+    e.flipped = true;
     PG_LAUNCH(c, "metric_flip", eblocks(3 * nn), kEBlock, k_flip, nn, e.csi.as<double>(), e.eta.as<double>(),
               e.zet.as<double>(), e.aj.as<double>());
     PG_CHECK(minmax(&lo, &hi));

@@ -1066,7 +1066,10 @@ int picgpu_step(picgpu_ctx *c, double dt, int nsteps, unsigned flags) {
     return PICGPU_E_STATE;
   }
   c->steps_since_compact = 0;
+  const bool fields = (flags & PICGPU_STEP_FIELDS) != 0;
+  if (fields) PG_CHECK(eulerian_fields_begin(c));
   for (int s = 0; s < nsteps; ++s) {
+    if (fields) PG_CHECK(eulerian_fields_async(c));
     if (!(flags & PICGPU_STEP_STAGED)) {
       c->stats.particle_steps += (s == 0) ? c->n : c->n_live;
       PG_CHECK(do_center_to_corner(c));
@@ -1091,6 +1094,7 @@ int picgpu_step(picgpu_ctx *c, double dt, int nsteps, unsigned flags) {
     if (flags & PICGPU_STEP_DEPOSIT_CELLS) PG_CHECK(do_scatter_P(c));
     if (flags & PICGPU_STEP_DEPOSIT_FACES) PG_CHECK(do_deposit_faces(c));
   }
+  if (fields) PG_CHECK(eulerian_fields_end(c));
   PG_HIP(hipStreamSynchronize(c->stream));
   return PICGPU_OK;
 }

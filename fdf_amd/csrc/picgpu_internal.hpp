@@ -128,6 +128,7 @@ struct picgpu_ctx {
     bool have_metrics = false;     // metrics computed for the current grid
     bool have_bc = false;          // boundary system created
     int orientation = 1;           // GridOrientation
+    bool flipped = false;          // a left-handed grid's metrics were negated
     picgpu_boundary_config bc{};
   } eul;
 
@@ -202,6 +203,12 @@ int scan_exclusive_i32(picgpu_ctx *c, const int32_t *in, int32_t *out, int64_t n
 void eulerian_release(picgpu_ctx *c);
 // device pointer and element count of an Eulerian node field (PICGPU_FIELD_UCAT..AJ)
 int eulerian_field(picgpu_ctx *c, int which, const double **d_ptr, int64_t *count);
+// SetEulerianFields of a step after the first, launched without waiting;
+// eulerian_fields_begin checks the state and clears the error report,
+// eulerian_fields_end waits and reports a singular cell system
+int eulerian_fields_begin(picgpu_ctx *c);
+int eulerian_fields_async(picgpu_ctx *c);
+int eulerian_fields_end(picgpu_ctx *c);
 // keeps the Eulerian Ucat (when metrics exist) equal to a freshly uploaded
 // node-slot Ucat; kind: hipMemcpyHostToDevice or hipMemcpyDeviceToDevice
 int eulerian_take_ucat(picgpu_ctx *c, const double *ucat, hipMemcpyKind kind);

@@ -180,9 +180,12 @@ class PicGpu:
         L.check(self.lib.picgpu_deposit_faces(self.h, particle_volume))
 
     def AdvanceSimulation(self, dt: float, nsteps: int, rk2: bool = False, deposit_cells: bool = True,
-                          deposit_faces: bool = False, staged: bool = False):
+                          deposit_faces: bool = False, staged: bool = False, fields: bool = False):
+        """fields=True: every step first prepares Ucat (boundaries, Contra2Cart,
+        boundaries), as SetEulerianFields does for the steps after the first."""
         flags = (L.STEP_RK2 if rk2 else 0) | (L.STEP_DEPOSIT_CELLS if deposit_cells else 0) | \
-                (L.STEP_DEPOSIT_FACES if deposit_faces else 0) | (L.STEP_STAGED if staged else 0)
+                (L.STEP_DEPOSIT_FACES if deposit_faces else 0) | (L.STEP_STAGED if staged else 0) | \
+                (L.STEP_FIELDS if fields else 0)
         L.check(self.lib.picgpu_step(self.h, dt, nsteps, flags))
 
     # ---- Eulerian field preparation ----
@@ -205,6 +208,9 @@ class PicGpu:
         ucont = np.ascontiguousarray(ucont, dtype=np.float64)
         assert ucont.size == 3 * (self.IM + 1) * (self.JM + 1) * (self.KM + 1)
         L.check(self.lib.picgpu_upload_ucont(self.h, _ptr(ucont)))
+
+    def upload_ucont_device(self, d_ucont: int):
+        L.check(self.lib.picgpu_upload_ucont_device(self.h, C.c_void_p(d_ucont)))
 
     def SetInitialInteriorField(self, finit: int, uin=(0.0, 0.0, 0.0)):
         u = np.ascontiguousarray(uin, np.float64)

@@ -355,3 +355,28 @@ def config(c: int, scale: float = 1.0, xp=None, with_particles: bool = True) -> 
         else:
             d["pos"] = particles_uniform_tube(n, 1000 + c)
     return d
+
+
+def contravariant_from_cartesian(coor, ucat, IM, JM, KM):
+    """Ucont [(KM+1)(JM+1)(IM+1)][3] of a node-slot Cartesian field: the flux
+    through the i-face stored at node (i,j,k) (nodes i, j-1..j, k-1..k) is
+    that face's area vector, half the cross product of its diagonals, dotted
+    with ucat at the same slot; likewise j and k.  Face slots whose face is
+    undefined (j = 0 or k = 0 for i-faces, etc.) get 0.  Contra2Cart of the
+    result returns ucat to within the grid's variation over a cell, which is
+    what the per-step field preparation of a synthetic workload needs.
+    Works on numpy arrays and torch tensors alike."""
+    N = coor.reshape(KM + 1, JM + 1, IM + 1, 3)
+    U = ucat.reshape(KM + 1, JM + 1, IM + 1, 3)
+    out = 0.0 * U  # same type, device and shape
+
+    def cross_dot(d1, d2, u):
+        cx = d1[..., 1] * d2[..., 2] - d1[..., 2] * d2[..., 1]
+        cy = d1[..., 2] * d2[..., 0] - d1[..., 0] * d2[..., 2]
+        cz = d1[..., 0] * d2[..., 1] - d1[..., 1] * d2[..., 0]
+        return 0.5 * (cx * u[..., 0] + cy * u[..., 1] + cz * u[..., 2])
+
+    out[1:, 1:, :, 0] = cross_dot(N[1:, 1:, :] - N[:-1, :-1, :], N[1:, :-1, :] - N[:-1, 1:, :], U[1:, 1:, :])
+    out[1:, :, 1:, 1] = cross_dot(N[1:, :, 1:] - N[:-1, :, :-1], N[:-1, :, 1:] - N[1:, :, :-1], U[1:, :, 1:])
+    out[:, 1:, 1:, 2] = cross_dot(N[:, 1:, 1:] - N[:, :-1, :-1], N[:, 1:, :-1] - N[:, :-1, 1:], U[:, 1:, 1:])
+    return out.reshape(-1, 3)

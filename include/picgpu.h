@@ -81,6 +81,9 @@ extern "C" {
 #define PICGPU_STEP_RK2 0x1           /* Heun predictor-corrector instead of forward Euler */
 #define PICGPU_STEP_DEPOSIT_CELLS 0x2 /* ParticleCount + P average */
 #define PICGPU_STEP_DEPOSIT_FACES 0x4 /* face count + momentum */
+#define PICGPU_STEP_FIELDS 0x8        /* each step first prepares Ucat as SetEulerianFields does for a
+                                         step after the first (simulation.c:78-92: boundaries,
+                                         Contra2Cart, boundaries); needs picgpu_boundary_create */
 #define PICGPU_STEP_STAGED 0x100      /* run the stage kernels one by one (reference stage order)
                                          instead of the fused one-pass step; same results */
 

@@ -280,3 +280,37 @@ def test_full_size_properties(swarm_cls):
     assert clo_max <= 1e-12 * cmax
     # interior flow is along +z, mostly
     assert np.median(u[1:-1, 1:-1, 1:-1, 2]) > 0.5
+
+
+def test_step_flag_prepares_fields_every_step(swarm_cls):
+    """PICGPU_STEP_FIELDS runs the preparation before each step; the result
+    equals preparing by hand before each single step (FP64, per PID)."""
+    _say("eulerian step flag")
+    IM, JM, KM = 16, 16, 40
+    coor = W.tube_grid(IM, JM, KM)
+    cfg = channel(4, "constant_velocity", 1.0)
+    pos = W.particles_uniform_tube(20000, 78)
+    outs = []
+    for by_hand in (False, True):
+        with swarm_cls(precision="fp64") as g:
+            g.upload_grid(IM, JM, KM, coor)
+            g.compute_metrics()
+            g.BoundarySystem_Create(cfg)
+            g.SetEulerianFields(0, 0, 1, (0.0, 0.0, 1.0))
+            g.upload_particles(pos)
+            g.LocateAllParticlesInGrid()
+            g.InterpolateAllFieldsToSwarm()
+            if by_hand:
+                for s in range(3):
+                    g.SetEulerianFields(s + 1, 0)
+                    g.AdvanceSimulation(0.05, 1, rk2=True, deposit_faces=True)
+            else:
+                g.AdvanceSimulation(0.05, 3, rk2=True, deposit_faces=True, fields=True)
+            d = g.download_particles()
+            o = np.argsort(d["pid"])
+            outs.append(({k: v[o] for k, v in d.items()}, g.field(L.FIELD_UCAT), g.field(L.FIELD_KFACE)))
+    (a, ua, fa), (b, ub, fb) = outs
+    assert len(a["pid"]) > 5000
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(ua, ub) and np.array_equal(fa, fb)
