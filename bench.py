@@ -210,11 +210,12 @@ def run_variant(name: str, d: dict, steps: int, warmup: int, mlp: dict | None):
             kernels[k] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt}
     res["kernels"] = kernels
     if fields and "contra2cart" in kernels:
-        # Contra2Cart per interior cell: Ucont, Csi, Eta, Zet read (4 x 24 B),
-        # Ucat written (24 B) and its cell-linear copy (3 reals)
+        # Contra2Cart per interior cell: node coordinates and Ucont read
+        # (2 x 24 B; the face vectors are functions of the coordinates), Ucat
+        # written (24 B) and its cell-linear copy (3 reals)
         rs = 8 if precision == "fp64" else 4
         ncell_in = (IM - 1) * (JM - 1) * (KM - 1)
-        byts = ncell_in * (4 * 24 + 24 + 3 * rs)
+        byts = ncell_in * (2 * 24 + 24 + 3 * rs)
         avg = kernels["contra2cart"]["avg_ms"]
         res["fields"] = {"contra2cart_avg_ms": avg, "algorithmic_bytes": byts,
                          "achieved_GBs": byts / (avg * 1e-3) / 1e9,
@@ -357,6 +358,40 @@ def cpu_baseline(d: dict, sample_particles: int, sample_steps: int) -> dict:
                       f"FP64 oracle -O3 x86-64-v3 OpenMP, {dt:.1f} s"}
 
 
+def fields_cpu_baseline(d: dict, kplanes: int = 64, reps: int = 3) -> dict:
+    """Per-step field preparation (boundaries, Contra2Cart, boundaries) by the
+    FP64 oracle built -O3 with OpenMP, on the first `kplanes` cell planes of
+    the same grid and Ucont (metrics computed untimed)."""
+    import numpy as np
+
+    from oracle.oracle import Oracle
+
+    threads = min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    o = Oracle(fast=True)
+    IM, JM, KM = d["IM"], d["JM"], min(kplanes, d["KM"])
+    nn = (IM + 1) * (JM + 1) * (KM + 1)
+    coor = d["coor"].reshape(-1, 3)[:nn].cpu().numpy()
+    ucont = d["ucont"].reshape(-1, 3)[:nn].cpu().numpy().copy()
+    csi, eta, zet, _, ori = o.metrics(IM, JM, KM, coor)
+
+    class _Nograd:
+        type = [1] * 6
+        handler = [1] * 6
+        vel = [[0.0, 0.0, 0.0]] * 6
+        vmax = [1.0] * 6
+
+    ucat = np.zeros_like(ucont)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        o.set_eulerian_fields(IM, JM, KM, _Nograd, csi, eta, zet, 1, 0, (0.0, 0.0, 0.0), ori, ucat, ucont)
+    dt = (time.perf_counter() - t0) / reps
+    ncell_in = (IM - 1) * (JM - 1) * (KM - 1)
+    return {"value": ncell_in / dt, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": f"first {KM} of {d['KM']} cell planes ({IM}x{JM}x{KM}), {reps} preparations, "
+                      f"FP64 oracle -O3 x86-64-v3 OpenMP, {dt * 1e3:.1f} ms each"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -437,6 +472,15 @@ def main():
             log(f"cpu baseline: {cpu['value']:.4g} particle-steps/s on {cpu['cores']} cores")
         except Exception as e:
             log("cpu baseline failed:", e)
+        if "ucont" in d:
+            try:
+                fc = fields_cpu_baseline(d)
+                log(f"field preparation cpu baseline: {fc['value']:.4g} cells/s on {fc['cores']} cores")
+                for r in results.values():
+                    if "fields" in r:
+                        r["fields"]["cpu_baseline"] = fc
+            except Exception as e:
+                log("field preparation cpu baseline failed:", e)
     line = {
         "metric": METRIC,
         "value": head["value"],

@@ -237,11 +237,14 @@ static void apply_face(const dims_t *d, const ref_bc_t *bc, int face, const doub
   const int64_t u = pos ? p - 1 : p;
   const double *metric = axis == 0 ? csi : axis == 1 ? eta : zet;
   const int h = bc->handler[face];
-  for (int64_t k = 0; k < d->mz; ++k)
-    for (int64_t j = 0; j < d->my; ++j)
-      for (int64_t i = 0; i < d->mx; ++i) {
+  /* only the face plane: the loop bound of the face axis is pinned to p */
+  const int64_t klo = axis == 2 ? p : 0, khi = axis == 2 ? p + 1 : d->mz;
+  const int64_t jlo = axis == 1 ? p : 0, jhi = axis == 1 ? p + 1 : d->my;
+  const int64_t ilo = axis == 0 ? p : 0, ihi = axis == 0 ? p + 1 : d->mx;
+  for (int64_t k = klo; k < khi; ++k)
+    for (int64_t j = jlo; j < jhi; ++j)
+      for (int64_t i = ilo; i < ihi; ++i) {
         int64_t idx[3] = {i, j, k};
-        if (idx[axis] != p) continue;
         const int64_t np = at(d, i, j, k);
         idx[axis] = q;
         const int64_t nq = at(d, idx[0], idx[1], idx[2]);
