@@ -230,6 +230,21 @@ def run_variant(name: str, d: dict, steps: int, warmup: int, mlp: dict | None):
     res["walk_iterations_per_particle_step"] = walk_percentiles(st["walk_hist"])
     res["walk_iterations_per_particle_step"]["mean"] = st["walk_iterations"] / max(1, int(st["walk_hist"].sum()))
     g.walk_stats(False)
+    if d.get("io_dir"):
+        # output files of one step (PETSc binary Vec files), then the restart
+        # read of the same files; bytes = file payloads
+        n_now = g.n
+        nn = (IM + 1) * (JM + 1) * (KM + 1)
+        byts = 2 * 24 * n_now + (2 * 24 + 3 * 8) * nn
+        t0 = time.perf_counter()
+        g.WriteOutput(1, d["io_dir"])
+        t1 = time.perf_counter()
+        g.ReadRestart(1, d["io_dir"])
+        t2 = time.perf_counter()
+        res["io"] = {"bytes": byts, "write_s": t1 - t0, "read_s": t2 - t1,
+                     "write_GBs": byts / (t1 - t0) / 1e9, "read_GBs": byts / (t2 - t1) / 1e9,
+                     "directory": d["io_dir"]}
+        log(f"{name}: output {byts / 1e9:.2f} GB written in {t1 - t0:.2f} s, read back in {t2 - t1:.2f} s")
     g.close()
     log(f"{name}: {res['value']:.4g} particle-steps/s, {res['ms_per_step']:.3f} ms/step")
     return res
@@ -408,6 +423,9 @@ def main():
     ap.add_argument("--fields", action="store_true",
                     help="every step first prepares Ucat from Ucont (boundaries, Contra2Cart, boundaries), "
                          "as the reference's SetEulerianFields does each step")
+    ap.add_argument("--io", default=None, metavar="DIR",
+                    help="after the timed steps, write one step's output files to DIR and read them back "
+                         "as a restart (timed separately, not part of the headline)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -427,6 +445,9 @@ def main():
     t0 = time.perf_counter()
     d = make_inputs(args.config, args.scale, dev)
     d["workload_name"] = f"config{args.config} scale={args.scale:g}"
+    if args.io:
+        os.makedirs(args.io, exist_ok=True)
+        d["io_dir"] = args.io
     if args.fields:
         from fdf_amd import workloads as W
 

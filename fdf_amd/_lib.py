@@ -132,6 +132,15 @@ _SIG = {
     "picgpu_boundary_execute": (C.c_int, [_P]),
     "picgpu_contra2cart": (C.c_int, [_P]),
     "picgpu_set_eulerian_fields": (C.c_int, [_P, _I64, _I64, C.c_int, _P]),
+    "picgpu_read_grid_file": (C.c_int, [C.c_char_p, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64),
+                                        C.POINTER(_I64), _P]),
+    "picgpu_write_grid_file": (C.c_int, [C.c_char_p, _I64, _I64, _I64, _P]),
+    "picgpu_field_file_name": (C.c_int, [C.c_char_p, C.c_size_t, C.c_char_p, C.c_char_p, _I64, C.c_char_p]),
+    "picgpu_petsc_vec_length": (C.c_int, [C.c_char_p, C.POINTER(_I64)]),
+    "picgpu_read_petsc_vec": (C.c_int, [C.c_char_p, _I64, _P]),
+    "picgpu_write_petsc_vec": (C.c_int, [C.c_char_p, _I64, C.c_int, _P]),
+    "picgpu_write_output": (C.c_int, [_P, C.c_char_p, _I64]),
+    "picgpu_read_restart": (C.c_int, [_P, C.c_char_p, _I64]),
     "picgpu_mlp_param_count": (_I64, []),
     "picgpu_mlp_set_params": (C.c_int, [_P, _P]),
     "picgpu_mlp_get_params": (C.c_int, [_P, _P]),

@@ -226,6 +226,15 @@ class PicGpu:
         u = np.ascontiguousarray(uin, np.float64)
         L.check(self.lib.picgpu_set_eulerian_fields(self.h, int(step), int(start_step), int(finit), _ptr(u)))
 
+    # ---- on-disk formats (PETSc binary Vec files under `directory`) ----
+    def WriteOutput(self, step: int, directory: str = "results"):
+        """WriteSwarmField(position, velocity) + WriteSimulationFields (simulation.c:304-306)."""
+        L.check(self.lib.picgpu_write_output(self.h, directory.encode(), int(step)))
+
+    def ReadRestart(self, step: int, directory: str = "results"):
+        """ReadSimulationFields + ReadAllSwarmFields + PrepareLoadedSwarmForRelocation."""
+        L.check(self.lib.picgpu_read_restart(self.h, directory.encode(), int(step)))
+
     def synchronize(self):
         L.check(self.lib.picgpu_synchronize(self.h))
 

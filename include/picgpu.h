@@ -250,6 +250,34 @@ int picgpu_contra2cart(picgpu_ctx *ctx);
  * otherwise [step 0: initial Ucont], boundaries, Contra2Cart, boundaries. */
 int picgpu_set_eulerian_fields(picgpu_ctx *ctx, int64_t step, int64_t start_step, int finit, const double uin[3]);
 
+/* ---- on-disk formats: PETSc binary Vec files and the FDFGRID grid file ----
+ * Host-side; the Vec files are PETSc 3.19's binary format (big-endian
+ * VEC_FILE_CLASSID 1211214 and length as 64-bit PetscInt, then big-endian
+ * doubles, plus "<file>.info"); 32-bit-PetscInt headers are read too.
+ *   picgpu_read_grid_file   <- ReadGridFile io.c:101-176 + the file read of
+ *                              AssignGridCoordinates grid.c:356-383
+ *   picgpu_write_grid_file  <- none (writes the README.md:134-146 format)
+ *   picgpu_field_file_name  <- the "results/<field>%05d_0.<ext>" names, io.c:680-682
+ *   picgpu_petsc_vec_length <- the VecLoad size probe, ParticleMotion.c:1200-1228
+ *   picgpu_read_petsc_vec   <- ReadFieldData io.c:253-436 (one rank)
+ *   picgpu_write_petsc_vec  <- WriteFieldData io.c:657-745 (one rank)
+ *   picgpu_write_output     <- WriteSwarmField(position, velocity) + WriteSimulationFields
+ *                              simulation.c:304-306, io.c:887-936
+ *   picgpu_read_restart     <- ReadSimulationFields io.c:437-488, ReadAllSwarmFields
+ *                              io.c:1974-2057, PrepareLoadedSwarmForRelocation
+ *                              ParticleSwarm.c:1073-1130
+ * dir NULL or "" means "results".  Missing or mis-sized files: PICGPU_E_ARG. */
+/* sizes of block 0 (and the block count); coor [(KM+1)(JM+1)(IM+1)][3] may be NULL */
+int picgpu_read_grid_file(const char *path, int64_t *IM, int64_t *JM, int64_t *KM, int64_t *nblk, double *coor);
+int picgpu_write_grid_file(const char *path, int64_t IM, int64_t JM, int64_t KM, const double *coor);
+int picgpu_field_file_name(char *buf, size_t size, const char *dir, const char *field, int64_t step,
+                           const char *ext);
+int picgpu_petsc_vec_length(const char *path, int64_t *len);
+int picgpu_read_petsc_vec(const char *path, int64_t len, double *out);
+int picgpu_write_petsc_vec(const char *path, int64_t len, int block_size, const double *data);
+int picgpu_write_output(picgpu_ctx *ctx, const char *dir, int64_t step);
+int picgpu_read_restart(picgpu_ctx *ctx, const char *dir, int64_t step);
+
 /* ---- MLP initial guess (3 -> 64 -> 64 -> 3, GELU, BF16 weights, FP32 accumulation) ---- */
 #define PICGPU_MLP_HIDDEN 64
 /* number of FP32 parameters: W1[64][3] b1[64] W2[64][64] b2[64] W3[3][64] b3[3] */
