@@ -226,6 +226,16 @@ class PicGpu:
         u = np.ascontiguousarray(uin, np.float64)
         L.check(self.lib.picgpu_set_eulerian_fields(self.h, int(step), int(start_step), int(finit), _ptr(u)))
 
+    # ---- seeding (the reference's drand48 stream; seed 202403 + rank there) ----
+    def InitializeParticleBasicProperties(self, n: int, volumetric: bool = True, inlet_face: int = 4,
+                                          seed: int = 202403, pid_base: int = 0):
+        """n particles in the volume (-pinit 1) or on the inlet face (-pinit 0)."""
+        L.check(self.lib.picgpu_seed_particles(self.h, int(n), 1 if volumetric else 0, int(inlet_face), int(seed),
+                                               int(pid_base)))
+
+    def ReinitializeParticlesOnInletSurface(self, inlet_face: int = 4, seed: int = 202403):
+        L.check(self.lib.picgpu_reinit_on_inlet(self.h, int(inlet_face), int(seed)))
+
     # ---- on-disk formats (PETSc binary Vec files under `directory`) ----
     def WriteOutput(self, step: int, directory: str = "results"):
         """WriteSwarmField(position, velocity) + WriteSimulationFields (simulation.c:304-306)."""

@@ -278,6 +278,23 @@ int picgpu_write_petsc_vec(const char *path, int64_t len, int block_size, const 
 int picgpu_write_output(picgpu_ctx *ctx, const char *dir, int64_t step);
 int picgpu_read_restart(picgpu_ctx *ctx, const char *dir, int64_t step);
 
+/* ---- particle seeding, drawing the reference's random stream ----
+ *   picgpu_seed_particles  <- InitializeParticleBasicProperties ParticleSwarm.c:340-499
+ *                             (volume: DetermineVolumetricInitializationParameters :229-338;
+ *                              inlet surface: GetRandomCellAndLogicOnInletFace Boundaries.c:224-390;
+ *                              position: MetricLogicalToPhysical Metric.c:68-91)
+ *   picgpu_reinit_on_inlet <- ReinitializeParticlesOnInletSurface ParticleMotion.c:1393-1517
+ * The stream is the reference's PETSCRAND48 one: drand48 after srand48(seed);
+ * the reference seeds it with 202403 + rank (ParticleSwarm.c:218).  Particle p
+ * takes draws 6p..6p+5 (volume) or 4p..4p+3 (inlet face).  Seeded particles
+ * are UNINITIALIZED with no cell, velocity 0, P 101325, PID pid_base + p;
+ * the re-initialisation moves the current particles (in their array order)
+ * onto the face and clears their cells, statuses unchanged. */
+#define PICGPU_SEED_SURFACE 0 /* -pinit 0: on the inlet face */
+#define PICGPU_SEED_VOLUME 1  /* -pinit 1: anywhere in the grid */
+int picgpu_seed_particles(picgpu_ctx *ctx, int64_t n, int mode, int inlet_face, int64_t seed, int64_t pid_base);
+int picgpu_reinit_on_inlet(picgpu_ctx *ctx, int inlet_face, int64_t seed);
+
 /* ---- MLP initial guess (3 -> 64 -> 64 -> 3, GELU, BF16 weights, FP32 accumulation) ---- */
 #define PICGPU_MLP_HIDDEN 64
 /* number of FP32 parameters: W1[64][3] b1[64] W2[64][64] b2[64] W3[3][64] b3[3] */

@@ -90,6 +90,9 @@ class Oracle:
             "ref_bc_execute": (I, [I64, I64, I64, C.POINTER(_BC), P, P, P, P, P]),
             "ref_contra2cart": (I, [I64, I64, I64, P, P, P, P, P]),
             "ref_set_eulerian_fields": (I, [I64, I64, I64, C.POINTER(_BC), P, P, P, I64, I, P, I, P, P]),
+            "ref_srand48_state": (C.c_uint64, [I64]),
+            "ref_drand48": (D, [C.POINTER(C.c_uint64)]),
+            "ref_seed_particles": (None, [I64, I64, I64, P, I64, I, I, I64, P]),
             "ref_mlp_param_count": (I64, [I]),
             "ref_mlp_forward": (None, [I, P, I64, P, P, I]),
             "ref_mlp_loss_grad": (D, [I, P, I64, P, P, P, I]),
@@ -257,6 +260,17 @@ class Oracle:
                                              int(finit), _p(uin), int(orientation), _p(ucat), _p(ucont))
         if r != 0:
             raise RuntimeError(f"oracle eulerian error {r}")
+
+    # ---- seeding ----
+    def drand48_stream(self, seed: int, count: int) -> np.ndarray:
+        x = C.c_uint64(self.lib.ref_srand48_state(int(seed)))
+        return np.array([self.lib.ref_drand48(C.byref(x)) for _ in range(count)])
+
+    def seed_particles(self, IM, JM, KM, coor, n, volumetric=True, face=4, seed=202403):
+        coor = np.ascontiguousarray(coor, np.float64)
+        pos = np.zeros((n, 3))
+        self.lib.ref_seed_particles(IM, JM, KM, _p(coor), n, 1 if volumetric else 0, int(face), int(seed), _p(pos))
+        return pos
 
     # ---- MLP (spec) ----
     def mlp_forward(self, params, x, round_bf16=True, H=64):

@@ -160,6 +160,15 @@ int ref_set_eulerian_fields(int64_t IM, int64_t JM, int64_t KM, const ref_bc_t *
                             const double *eta, const double *zet, int64_t step, int finit, const double uin[3],
                             int orientation, double *ucat, double *ucont);        /* simulation.c:38-107 */
 
+/* ---- particle seeding (seedref.c) ---- */
+uint64_t ref_srand48_state(int64_t seed);   /* POSIX srand48 */
+double ref_drand48(uint64_t *state);        /* POSIX drand48 */
+/* n particles from the stream srand48(seed): mode 1 volumetric
+ * (ParticleSwarm.c:229-338), mode 0 on inlet face `face`
+ * (Boundaries.c:224-390); positions by MetricLogicalToPhysical (Metric.c:68-91) */
+void ref_seed_particles(int64_t IM, int64_t JM, int64_t KM, const double *coor, int64_t n, int mode, int face,
+                        int64_t seed, double *pos);
+
 /* ---- SPEC: MLP initial guess, 3 -> H -> H -> 3, GELU(erf) ---- */
 /* params layout: W1[H][3], b1[H], W2[H][H], b2[H], W3[3][H], b3[3] (row = output) */
 int64_t ref_mlp_param_count(int H);
