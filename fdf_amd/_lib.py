@@ -141,6 +141,9 @@ _SIG = {
     "picgpu_write_petsc_vec": (C.c_int, [C.c_char_p, _I64, C.c_int, _P]),
     "picgpu_write_output": (C.c_int, [_P, C.c_char_p, _I64]),
     "picgpu_read_restart": (C.c_int, [_P, C.c_char_p, _I64]),
+    "picgpu_write_vts": (C.c_int, [C.c_char_p, _I64, _I64, _I64, _P, C.c_char_p, C.c_int, _P]),
+    "picgpu_write_vtp": (C.c_int, [C.c_char_p, _I64, _P, C.c_char_p, C.c_int, _P]),
+    "picgpu_write_vtk": (C.c_int, [_P, C.c_char_p, _I64]),
     "picgpu_seed_particles": (C.c_int, [_P, _I64, C.c_int, C.c_int, _I64, _I64]),
     "picgpu_reinit_on_inlet": (C.c_int, [_P, C.c_int, _I64]),
     "picgpu_mlp_param_count": (_I64, []),

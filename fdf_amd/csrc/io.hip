@@ -350,3 +350,174 @@ int picgpu_read_restart(picgpu_ctx *c, const char *dir, int64_t step) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// VTK XML output (io.c:1264-1815, postprocess.c:63-520): appended raw
+// little-endian blocks, each led by its byte count as the reference's
+// 64-bit PetscInt, offsets counted from the byte after "_".  The one
+// difference from the reference's bytes: the VTKFile element also says
+// header_type="UInt64", which is what the 8-byte counts need for VTK readers
+// (without it they take 4-byte counts and misread every array).
+// ---------------------------------------------------------------------------
+namespace {
+
+bool put_block(FILE *f, const void *data, int64_t count, size_t elem) {
+  const int64_t bytes = count * (int64_t)elem;  // WriteVTKAppendedBlock, io.c:1264-1275
+  if (fwrite(&bytes, sizeof(bytes), 1, f) != 1) return false;
+  return count == 0 || fwrite(data, elem, (size_t)count, f) == (size_t)count;
+}
+
+int write_vts(const char *path, int64_t IM, int64_t JM, int64_t KM, const double *coor, const char *name, int ncomp,
+              const double *field) {
+  File fp(fopen(path, "wb"));
+  if (!fp.f) return io_error("cannot create", path);
+  const int64_t mx = IM + 1, my = JM + 1, mz = KM + 1, nn = mx * my * mz;
+  int64_t off = 0;
+  FILE *f = fp.f;
+  fprintf(f, "<?xml version=\"1.0\"?>\n");
+  fprintf(f, "<VTKFile type=\"StructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\" header_type=\"UInt64\">\n");
+  fprintf(f, "  <StructuredGrid WholeExtent=\"%d %" PRId64 " %d %" PRId64 " %d %" PRId64 "\">\n", 0, mx - 1, 0, my - 1,
+          0, mz - 1);
+  fprintf(f, "    <Piece Extent=\"%d %" PRId64 " %d %" PRId64 " %d %" PRId64 "\">\n", 0, mx - 1, 0, my - 1, 0, mz - 1);
+  fprintf(f, "      <Points>\n");
+  fprintf(f,
+          "        <DataArray type=\"Float64\" Name=\"Position\" NumberOfComponents=\"3\" format=\"appended\" "
+          "offset=\"%" PRId64 "\" />\n",
+          off);
+  off += 8 + 3 * nn * 8;
+  fprintf(f, "      </Points>\n");
+  fprintf(f, "      <PointData Scalars=\"%s\">\n", name);
+  fprintf(f,
+          "        <DataArray type=\"Float64\" Name=\"%s\" NumberOfComponents=\"%d\" format=\"appended\" "
+          "offset=\"%" PRId64 "\" />\n",
+          name, ncomp, off);
+  fprintf(f, "      </PointData>\n");
+  fprintf(f, "    </Piece>\n");
+  fprintf(f, "  </StructuredGrid>\n");
+  fprintf(f, "  <AppendedData encoding=\"raw\">\n");
+  fprintf(f, "_");
+  if (!put_block(f, coor, 3 * nn, sizeof(double)) || !put_block(f, field, (int64_t)ncomp * nn, sizeof(double)))
+    return io_error("write failed", path);
+  if (fprintf(f, "\n </AppendedData>\n</VTKFile>\n") < 0) return io_error("write failed", path);
+  return PICGPU_OK;
+}
+
+int write_vtp(const char *path, int64_t n, const double *pos, const char *name, int ncomp, const double *field) {
+  File fp(fopen(path, "wb"));
+  if (!fp.f) return io_error("cannot create", path);
+  FILE *f = fp.f;
+  int64_t off = 0;
+  fprintf(f, "<?xml version=\"1.0\"?>\n");
+  fprintf(f, "<VTKFile type=\"PolyData\" version=\"1.0\" byte_order=\"LittleEndian\" header_type=\"UInt64\">\n");
+  fprintf(f, "  <PolyData>\n");
+  fprintf(f,
+          "    <Piece NumberOfPoints=\"%" PRId64 "\" NumberOfVerts=\"%" PRId64
+          "\" NumberOfLines=\"0\" NumberOfPolys=\"0\" NumberOfStrips=\"0\">\n",
+          n, n);
+  fprintf(f, "      <Points>\n");
+  fprintf(f,
+          "        <DataArray type=\"Float64\" Name=\"Position\" NumberOfComponents=\"3\" format=\"appended\" "
+          "offset=\"%" PRId64 "\"/>\n",
+          off);
+  off += 8 + n * 3 * 8;
+  fprintf(f, "      </Points>\n");
+  fprintf(f, "      <PointData>\n");
+  fprintf(f,
+          "        <DataArray type=\"Float64\" Name=\"%s\" NumberOfComponents=\"%d\" format=\"appended\" "
+          "offset=\"%" PRId64 "\"/>\n",
+          name, ncomp, off);
+  off += 8 + n * ncomp * 8;
+  fprintf(f, "      </PointData>\n");
+  fprintf(f, "      <Verts>\n");
+  fprintf(f, "        <DataArray type=\"Int64\" Name=\"connectivity\" format=\"appended\" offset=\"%" PRId64 "\"/>\n",
+          off);
+  off += 8 + n * 8;
+  fprintf(f, "        <DataArray type=\"Int64\" Name=\"offsets\" format=\"appended\" offset=\"%" PRId64 "\"/>\n", off);
+  fprintf(f, "      </Verts>\n");
+  fprintf(f, "    </Piece>\n");
+  fprintf(f, "  </PolyData>\n");
+  fprintf(f, "  <AppendedData encoding=\"raw\">\n");
+  fprintf(f, "_");
+  // one vertex per particle: connectivity i, offsets i + 1 (postprocess.c:205-213)
+  std::vector<int64_t> ids((size_t)n);
+  if (!put_block(f, pos, 3 * n, sizeof(double)) || !put_block(f, field, (int64_t)ncomp * n, sizeof(double)))
+    return io_error("write failed", path);
+  for (int64_t i = 0; i < n; ++i) ids[(size_t)i] = i;
+  if (!put_block(f, ids.data(), n, sizeof(int64_t))) return io_error("write failed", path);
+  for (int64_t i = 0; i < n; ++i) ids[(size_t)i] = i + 1;
+  if (!put_block(f, ids.data(), n, sizeof(int64_t))) return io_error("write failed", path);
+  if (fprintf(f, "\n  </AppendedData>\n</VTKFile>\n") < 0) return io_error("write failed", path);
+  return PICGPU_OK;
+}
+
+std::string vtk_path(const char *prefix, const char *name, int64_t step, const char *ext) {
+  char buf[512];
+  snprintf(buf, sizeof(buf), "%s_%05" PRId64 ".%s", name, step, ext);  // postprocess.c:296-297
+  std::string d = prefix && *prefix ? prefix : ".";
+  return d + "/" + buf;
+}
+
+}  // namespace
+
+extern "C" {
+
+int picgpu_write_vts(const char *path, int64_t IM, int64_t JM, int64_t KM, const double *coor, const char *name,
+                     int ncomp, const double *field) {
+  if (!path || !coor || !name || !field || IM <= 0 || JM <= 0 || KM <= 0 || (ncomp != 1 && ncomp != 3))
+    return PICGPU_E_ARG;
+  return write_vts(path, IM, JM, KM, coor, name, ncomp, field);
+}
+
+int picgpu_write_vtp(const char *path, int64_t n, const double *pos, const char *name, int ncomp,
+                     const double *field) {
+  if (!path || !name || n < 0 || (n > 0 && (!pos || !field)) || (ncomp != 1 && ncomp != 3)) return PICGPU_E_ARG;
+  return write_vtp(path, n, pos, name, ncomp, field);
+}
+
+// WriteEulerianVTK + WriteParticleVTK (postprocess.c:451-520): Ucat, P,
+// Ucont and ParticleCount on the grid nodes, the particles with their
+// velocity.  Fields this context has not computed are written as zeros.
+int picgpu_write_vtk(picgpu_ctx *c, const char *prefix, int64_t step) {
+  if (!c || step < 0) return PICGPU_E_ARG;
+  if (!c->have_grid) {
+    set_error("no grid uploaded");
+    return PICGPU_E_STATE;
+  }
+  const int64_t nn = c->nnode, n = c->n;
+  std::vector<double> coor((size_t)(3 * nn)), f((size_t)(3 * nn));
+  PG_HIP(hipMemcpy(coor.data(), c->coor.ptr, sizeof(double) * coor.size(), hipMemcpyDeviceToHost));
+  auto field = [&](int which, int64_t count, bool available) -> int {
+    if (available) return picgpu_download_grid_field(c, which, f.data());
+    memset(f.data(), 0, sizeof(double) * (size_t)count);
+    return PICGPU_OK;
+  };
+  const bool eul = c->eul.have_metrics;
+  PG_CHECK(field(PICGPU_FIELD_UCAT, 3 * nn, eul));
+  if (!eul && c->have_ucat) {  // the particle loop's cell-linear copy, in node slots
+    std::vector<unsigned char> h(c->rsize * 3 * (size_t)c->ncell);
+    PG_HIP(hipMemcpy(h.data(), c->ucat.ptr, h.size(), hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < c->KM; ++k)
+      for (int64_t j = 0; j < c->JM; ++j)
+        for (int64_t i = 0; i < c->IM; ++i) {
+          const int64_t cc = (k * c->JM + j) * c->IM + i, nd = (k * (c->JM + 1) + j) * (c->IM + 1) + i;
+          for (int q = 0; q < 3; ++q)
+            f[3 * nd + q] = c->rsize == 4 ? (double)((const float *)h.data())[3 * cc + q]
+                                          : ((const double *)h.data())[3 * cc + q];
+        }
+  }
+  PG_CHECK(write_vts(vtk_path(prefix, "Ucat", step, "vts").c_str(), c->IM, c->JM, c->KM, coor.data(), "Ucat", 3,
+                     f.data()));
+  PG_CHECK(field(PICGPU_FIELD_P_AVERAGE, nn, c->have_pavg));
+  PG_CHECK(write_vts(vtk_path(prefix, "P", step, "vts").c_str(), c->IM, c->JM, c->KM, coor.data(), "P", 1, f.data()));
+  PG_CHECK(field(PICGPU_FIELD_UCONT, 3 * nn, eul));
+  PG_CHECK(write_vts(vtk_path(prefix, "Ucont", step, "vts").c_str(), c->IM, c->JM, c->KM, coor.data(), "Ucont", 3,
+                     f.data()));
+  PG_CHECK(field(PICGPU_FIELD_PARTICLE_COUNT, nn, c->sorted || c->keyed));
+  PG_CHECK(write_vts(vtk_path(prefix, "ParticleCount", step, "vts").c_str(), c->IM, c->JM, c->KM, coor.data(),
+                     "ParticleCount", 1, f.data()));
+  std::vector<double> pos((size_t)(3 * n)), vel((size_t)(3 * n));
+  PG_CHECK(picgpu_download_particles(c, pos.data(), vel.data(), nullptr, nullptr, nullptr, nullptr));
+  return write_vtp(vtk_path(prefix, "velocity", step, "vtp").c_str(), n, pos.data(), "velocity", 3, vel.data());
+}
+
+}  // extern "C"

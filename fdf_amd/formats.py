@@ -59,3 +59,22 @@ def read_petsc_vec(path) -> np.ndarray:
 def write_petsc_vec(path, data, block_size: int = 1) -> None:
     data = np.ascontiguousarray(data, np.float64).ravel()
     L.check(L.load().picgpu_write_petsc_vec(_b(path), data.size, int(block_size), data.ctypes.data_as(C.c_void_p)))
+
+
+def write_vts(path, IM: int, JM: int, KM: int, coor, name: str, field) -> None:
+    """Structured-grid VTK file of one node field (1 or 3 components)."""
+    coor = np.ascontiguousarray(coor, np.float64).reshape(-1, 3)
+    field = np.ascontiguousarray(field, np.float64)
+    ncomp = field.size // coor.shape[0]
+    L.check(L.load().picgpu_write_vts(_b(path), IM, JM, KM, coor.ctypes.data_as(C.c_void_p), name.encode(), ncomp,
+                                      field.ctypes.data_as(C.c_void_p)))
+
+
+def write_vtp(path, pos, name: str, field) -> None:
+    """Poly-data VTK file of points with one point field (1 or 3 components)."""
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    field = np.ascontiguousarray(field, np.float64)
+    n = pos.shape[0]
+    ncomp = field.size // max(n, 1) if n else 3
+    L.check(L.load().picgpu_write_vtp(_b(path), n, pos.ctypes.data_as(C.c_void_p), name.encode(), ncomp,
+                                      field.ctypes.data_as(C.c_void_p)))

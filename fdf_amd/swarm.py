@@ -241,6 +241,10 @@ class PicGpu:
         """WriteSwarmField(position, velocity) + WriteSimulationFields (simulation.c:304-306)."""
         L.check(self.lib.picgpu_write_output(self.h, directory.encode(), int(step)))
 
+    def WriteVTK(self, step: int, prefix: str = "."):
+        """WriteEulerianVTK + WriteParticleVTK (postprocess.c:451-520)."""
+        L.check(self.lib.picgpu_write_vtk(self.h, prefix.encode(), int(step)))
+
     def ReadRestart(self, step: int, directory: str = "results"):
         """ReadSimulationFields + ReadAllSwarmFields + PrepareLoadedSwarmForRelocation."""
         L.check(self.lib.picgpu_read_restart(self.h, directory.encode(), int(step)))

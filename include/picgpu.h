@@ -277,6 +277,18 @@ int picgpu_read_petsc_vec(const char *path, int64_t len, double *out);
 int picgpu_write_petsc_vec(const char *path, int64_t len, int block_size, const double *data);
 int picgpu_write_output(picgpu_ctx *ctx, const char *dir, int64_t step);
 int picgpu_read_restart(picgpu_ctx *ctx, const char *dir, int64_t step);
+/* VTK XML output (io.c:1264-1815, postprocess.c:63-520): appended raw
+ * little-endian blocks with 8-byte (PetscInt) byte counts, the VTKFile element
+ * marked header_type="UInt64".  Structured grid: node coordinates and one
+ * node field of 1 or 3 components; poly data: points, one point field and one
+ * vertex per point.
+ *   picgpu_write_vtk <- WriteEulerianVTK + WriteParticleVTK postprocess.c:451-520:
+ *   <prefix>/Ucat_%05d.vts, P_, Ucont_, ParticleCount_ and velocity_%05d.vtp */
+int picgpu_write_vts(const char *path, int64_t IM, int64_t JM, int64_t KM, const double *coor, const char *name,
+                     int ncomp, const double *field);
+int picgpu_write_vtp(const char *path, int64_t n, const double *pos, const char *name, int ncomp,
+                     const double *field);
+int picgpu_write_vtk(picgpu_ctx *ctx, const char *prefix, int64_t step);
 
 /* ---- particle seeding, drawing the reference's random stream ----
  *   picgpu_seed_particles  <- InitializeParticleBasicProperties ParticleSwarm.c:340-499

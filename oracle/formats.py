@@ -84,3 +84,27 @@ def write_grid_file(path, IM, JM, KM, coor, header: bool = True) -> None:
         f.write(f"1\n{IM} {JM} {KM}\n")
         for x, y, z in coor.tolist():
             f.write(f"{x!r} {y!r} {z!r}\n")
+
+
+def read_vtk_xml(path):
+    """Arrays of a VTK XML file with appended raw data: {Name: ndarray}, plus the
+    header text.  Counts are read with the width header_type names (UInt32 if
+    absent), as VTK readers do; offsets count from the byte after "_"."""
+    import re
+
+    raw = open(path, "rb").read()
+    head, rest = raw.split(b'<AppendedData encoding="raw">', 1)
+    start = rest.index(b"_") + 1
+    blob = rest[start:]
+    text = head.decode()
+    hdr = re.search(r'header_type="(\w+)"', text)
+    cw = 8 if hdr and hdr.group(1) == "UInt64" else 4
+    arrays = {}
+    for m in re.finditer(r'<DataArray type="(\w+)" Name="(\w+)"(?: NumberOfComponents="(\d+)")? '
+                         r'format="appended" offset="(\d+)"', text):
+        typ, name, nc, off = m.group(1), m.group(2), int(m.group(3) or 1), int(m.group(4))
+        nbytes = int.from_bytes(blob[off:off + cw], "little")
+        dt = {"Float64": "<f8", "Int64": "<i8", "Int32": "<i4"}[typ]
+        a = np.frombuffer(blob[off + cw:off + cw + nbytes], dt)
+        arrays[name] = a.reshape(-1, nc) if nc > 1 else a
+    return arrays, text

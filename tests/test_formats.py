@@ -108,3 +108,46 @@ def test_grid_file_header_rules(tmp_path):
     assert (im, jm, km, nb) == (2, 2, 2, 2) and np.array_equal(c, coor)
     with pytest.raises(L.PicgpuError):
         F.read_grid_file(tmp_path / "missing.dat")
+
+
+VTS_HEAD = """<?xml version="1.0"?>
+<VTKFile type="StructuredGrid" version="0.1" byte_order="LittleEndian" header_type="UInt64">
+  <StructuredGrid WholeExtent="0 {mx} 0 {my} 0 {mz}">
+    <Piece Extent="0 {mx} 0 {my} 0 {mz}">
+      <Points>
+        <DataArray type="Float64" Name="Position" NumberOfComponents="3" format="appended" offset="0" />
+      </Points>
+      <PointData Scalars="{name}">
+        <DataArray type="Float64" Name="{name}" NumberOfComponents="{nc}" format="appended" offset="{off}" />
+      </PointData>
+    </Piece>
+  </StructuredGrid>
+  """
+
+
+@pytest.mark.parametrize("ncomp", [1, 3])
+def test_vts_layout_and_values(tmp_path, ncomp):
+    IM, JM, KM = 4, 3, 5
+    coor = W.perturbed_cube(IM, JM, KM, seed=2)
+    nn = (IM + 1) * (JM + 1) * (KM + 1)
+    field = np.random.default_rng(3).normal(size=(nn, ncomp)).squeeze()
+    p = tmp_path / "Ucat_00007.vts"
+    F.write_vts(p, IM, JM, KM, coor, "Ucat", field)
+    arrays, text = O.read_vtk_xml(p)
+    assert text == VTS_HEAD.format(mx=IM, my=JM, mz=KM, name="Ucat", nc=ncomp, off=8 + 3 * nn * 8)
+    assert np.array_equal(arrays["Position"], coor) and np.array_equal(arrays["Ucat"], field)
+    assert p.read_bytes().endswith(b"\n </AppendedData>\n</VTKFile>\n")
+
+
+def test_vtp_layout_and_values(tmp_path):
+    n = 1000
+    rng = np.random.default_rng(4)
+    pos, vel = rng.random((n, 3)), rng.normal(size=(n, 3))
+    p = tmp_path / "velocity_00003.vtp"
+    F.write_vtp(p, pos, "velocity", vel)
+    arrays, text = O.read_vtk_xml(p)
+    assert f'<Piece NumberOfPoints="{n}" NumberOfVerts="{n}" NumberOfLines="0"' in text
+    offs = [int(v) for v in __import__("re").findall(r'offset="(\d+)"', text)]
+    assert offs == [0, 8 + 24 * n, 2 * (8 + 24 * n), 2 * (8 + 24 * n) + 8 + 8 * n]
+    assert np.array_equal(arrays["Position"], pos) and np.array_equal(arrays["velocity"], vel)
+    assert np.array_equal(arrays["connectivity"], np.arange(n)) and np.array_equal(arrays["offsets"], np.arange(1, n + 1))

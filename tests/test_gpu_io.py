@@ -99,3 +99,29 @@ def test_restart_errors(swarm_cls, tmp_path):
         g.ReadRestart(1, str(tmp_path))
         assert g.n == 10
         assert (g.download_particles()["vel"] == 0).all()
+
+
+def test_write_vtk(swarm_cls, tmp_path):
+    _say("vtk output")
+    IM, JM, KM = 12, 10, 14
+    coor = W.perturbed_cube(IM, JM, KM, seed=23)
+    with swarm_cls(precision="fp32") as g:
+        g.upload_grid(IM, JM, KM, coor)
+        g.compute_metrics()
+        g.BoundarySystem_Create(channel(4, "constant_velocity", 1.0))
+        g.SetEulerianFields(0, 0, 1, (0.0, 0.0, 1.0))
+        g.InitializeParticleBasicProperties(5000, volumetric=True)
+        g.LocateAllParticlesInGrid()
+        g.InterpolateAllFieldsToSwarm()
+        g.AdvanceSimulation(0.01, 2, deposit_cells=True)
+        g.WriteVTK(2, str(tmp_path))
+        d = g.download_particles()
+        ucat, ucont = g.field(L.FIELD_UCAT), g.field(L.FIELD_UCONT)
+        pavg, count = g.field(L.FIELD_P_AVERAGE), g.field(L.FIELD_PARTICLE_COUNT)
+    for name, exp in (("Ucat", ucat.reshape(-1, 3)), ("Ucont", ucont.reshape(-1, 3)), ("P", pavg),
+                      ("ParticleCount", count)):
+        a, _ = O.read_vtk_xml(os.path.join(str(tmp_path), f"{name}_00002.vts"))
+        assert np.array_equal(a["Position"], coor) and np.array_equal(a[name], exp), name
+    a, _ = O.read_vtk_xml(os.path.join(str(tmp_path), "velocity_00002.vtp"))
+    assert np.array_equal(a["Position"], d["pos"]) and np.array_equal(a["velocity"], d["vel"])
+    assert np.array_equal(a["connectivity"], np.arange(len(d["pid"])))
