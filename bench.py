@@ -203,7 +203,7 @@ def run_variant(name: str, d: dict, steps: int, warmup: int, mlp: dict | None):
     for k in ("step_fused_fp64", "step_fused_fp32", "step_list_fp64", "step_list_fp32", "key_histogram",
               "key_scatter", "key_reindex", "step_deferred", "cell_planes", "bucket_pid_order", "iota", "gather_swarm", "locate_fp64", "locate_fp32",
               "interp", "center_to_corner", "rk2_predict", "rk2_correct", "keep_flags", "compact", "cell_histogram",
-              "sort_scatter", "scan_tile", "scan_add", "scatter_P", "deposit_faces", "reset_advect", "compact_tail",
+              "sort_scatter", "scan_reduce", "scan_tile", "scatter_P", "deposit_faces", "reset_advect", "compact_tail",
               "keep_unlocated", "contra2cart", "bc_face", "flag_fill"):
         ms, cnt = g.timing_query(k)
         if cnt:

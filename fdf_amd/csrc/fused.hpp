@@ -708,23 +708,41 @@ __global__ __launch_bounds__(kBlock) void k_key_scatter(int64_t n, const int32_t
 
 // After a step: idx_next[kstart_next[key(cell[p])] + slot[p]] = p for the
 // particles p = idx_cur[0..m) that survived it (slot >= 0; a slot in a
-// tile's own histogram is offset by the tile's base kbase[key]).
+// tile's own histogram is offset by the tile's base kbase[key]).  Each thread
+// takes kReindexPer entries, kBlock apart, and issues each round of its
+// dependent loads for all of them together.
+constexpr int kReindexPer = 4;
 __global__ __launch_bounds__(kBlock) void k_key_reindex(int64_t m, const int32_t *__restrict__ idx_cur,
                                                         const int32_t *__restrict__ cell,
                                                         const int32_t *__restrict__ slot,
                                                         const int32_t *__restrict__ kbase, CellIdx cx, TileMap tm,
                                                         const int32_t *__restrict__ kstart,
                                                         int32_t *__restrict__ idx_next) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= m) return;
-  const int32_t p = idx_cur[q];
-  int32_t s = slot[p];
-  if (s < 0) return;
-  int i, j, k;
-  unpack_cell(cell[p], cx, i, j, k);
-  const int32_t key = cell_key(tm, i, j, k);
-  if (s & kLocalSlot) s = kbase[key] + (s & (kLocalSlot - 1));
-  idx_next[kstart[key] + s] = p;
+  constexpr int NP = kReindexPer;
+  const int64_t q0 = (int64_t)blockIdx.x * (kBlock * NP) + threadIdx.x;
+  int32_t p[NP], s[NP], c[NP], key[NP], base[NP];
+#pragma unroll
+  for (int u = 0; u < NP; ++u) p[u] = q0 + u * kBlock < m ? idx_cur[q0 + u * kBlock] : -1;
+#pragma unroll
+  for (int u = 0; u < NP; ++u) {
+    s[u] = p[u] >= 0 ? slot[p[u]] : -1;
+    c[u] = p[u] >= 0 ? cell[p[u]] : 0;
+  }
+#pragma unroll
+  for (int u = 0; u < NP; ++u) {
+    key[u] = 0;
+    base[u] = 0;
+    if (s[u] >= 0) {
+      int i, j, k;
+      unpack_cell(c[u], cx, i, j, k);
+      key[u] = cell_key(tm, i, j, k);
+      base[u] = kstart[key[u]];
+      if (s[u] & kLocalSlot) base[u] += kbase[key[u]];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NP; ++u)
+    if (s[u] >= 0) idx_next[base[u] + (s[u] & (kLocalSlot - 1))] = p[u];
 }
 
 __global__ __launch_bounds__(kBlock) void k_iota(int64_t n, int32_t *__restrict__ idx) {
@@ -787,13 +805,16 @@ __global__ __launch_bounds__(kBlock) void k_bucket_pid_order(int64_t nkeys, cons
 // the lower neighbour's state and continues with its own particles (weight
 // 1-a).  A cell's first particles are read once, with independent loads, and
 // kept in registers for both passes; the sums are those of the
-// one-accumulator order above.
+// one-accumulator order above.  Neighbouring threads take neighbouring cells,
+// so that each load instruction of a wavefront reads a few contiguous lines.
+// In FP32 the kernel is held to 5 wavefronts per SIMD (at most 96 VGPRs, no
+// spills), which is what its LDS allows; FP64 needs more registers (4 per
+// SIMD).
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_deposit_tiles(int IM, int JM, int KM, TileMap tm, TileMap xm,
-                                                          const int32_t *__restrict__ kstart,
-                                                          const int32_t *__restrict__ idx, const R *__restrict__ w,
-                                                          const R *__restrict__ vel, R *__restrict__ fi,
-                                                          R *__restrict__ fj, R *__restrict__ fk) {
+__global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 5 : 1) void k_deposit_tiles(
+    int IM, int JM, int KM, TileMap tm, TileMap xm, const int32_t *__restrict__ kstart,
+    const int32_t *__restrict__ idx, const R *__restrict__ w, const R *__restrict__ vel, R *__restrict__ fi,
+    R *__restrict__ fj, R *__restrict__ fk) {
   static_assert(TS == kBlock, "one thread per cell of the tile");
   constexpr int NH = TY * TZ + TX * TZ + TX * TY;  // cells below the tile's i-, j- and k-faces
   constexpr bool COMP = sizeof(R) == 4;             // FP32 sums carry a compensation term
@@ -806,28 +827,97 @@ __global__ __launch_bounds__(kBlock) void k_deposit_tiles(int IM, int JM, int KM
   const int li = l % TX, lj = (l / TX) % TY, lk = l / (TX * TY);
   const int i = bi * TX + li, j = bj * TY + lj, k = bk * TZ + lk;
   const bool own = i < IM && j < JM && k < KM;
-  // weights and velocity of index position t
-  auto own_particle = [&](int32_t t, R a[3], R uu[3]) {
-    const int64_t q = idx[t];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      a[c] = w[3 * q + c];
-      uu[c] = vel[3 * q + c];
-    }
-  };
-  // the first K particles of the own cell are read once, with independent
-  // loads, and kept in registers for both passes
-  constexpr int K = 4;
-  int32_t s_own = 0, n_own = 0;
+  // The first K particles of the own cell (and of the cell below, for the
+  // threads that have one) are read once, in three rounds of independent
+  // loads (bucket bounds, indices, particle data) shared by both cells, and
+  // the own ones kept in registers for both passes; particles beyond the
+  // first K are read in batches of K.
+  constexpr int K = COMP ? 4 : 3;  // FP64: fewer, to stay at 4 wavefronts per SIMD
+  // the cell below the tile's lower face that thread l < NH sums, direction hd
+  int hi = -1, hj = -1, hk = -1, hd = 0;
+  if (l < TY * TZ) {
+    hd = 0, hi = bi * TX - 1, hj = bj * TY + l % TY, hk = bk * TZ + l / TY;
+  } else if (l < TY * TZ + TX * TZ) {
+    const int m = l - TY * TZ;
+    hd = 1, hi = bi * TX + m % TX, hj = bj * TY - 1, hk = bk * TZ + m / TX;
+  } else if (l < NH) {
+    const int m = l - TY * TZ - TX * TZ;
+    hd = 2, hi = bi * TX + m % TX, hj = bj * TY + m / TX, hk = bk * TZ - 1;
+  }
+  const bool halo = l < NH && hi >= 0 && hj >= 0 && hk >= 0 && hi < IM && hj < JM && hk < KM;
+  int32_t s_own = 0, n_own = 0, sh = 0, nh = 0;
   if (own) {
     const int32_t key = cell_key(tm, i, j, k);
     s_own = kstart[key];
     n_own = kstart[key + 1] - s_own;
   }
-  R cw[K][3], cv[K][3];
+  if (halo) {
+    const int32_t key = cell_key(tm, hi, hj, hk);
+    sh = kstart[key];
+    nh = kstart[key + 1] - sh;
+  }
+  int32_t qo[K], qh[K];
 #pragma unroll
-  for (int u = 0; u < K; ++u)
-    if (u < n_own) own_particle(s_own + u, cw[u], cv[u]);
+  for (int u = 0; u < K; ++u) {
+    qo[u] = u < n_own ? idx[s_own + u] : 0;
+    qh[u] = u < nh ? idx[sh + u] : 0;
+  }
+  R cw[K][3], cv[K][3], hw[K], hv[K][3];
+#pragma unroll
+  for (int u = 0; u < K; ++u) {
+    if (u < n_own) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        cw[u][c] = w[3 * (int64_t)qo[u] + c];
+        cv[u][c] = vel[3 * (int64_t)qo[u] + c];
+      }
+    }
+    if (u < nh) {
+      hw[u] = w[3 * (int64_t)qh[u] + hd];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) hv[u][c] = vel[3 * (int64_t)qh[u] + c];
+    }
+  }
+  // batch of K particles at bucket positions t0 .. t0+K-1 of a bucket [s, s+n)
+  auto load_batch = [&](int32_t s, int32_t n, int32_t t0, R bw[K][3], R bu[K][3]) {
+    int32_t qq[K];
+#pragma unroll
+    for (int u = 0; u < K; ++u) qq[u] = t0 + u < n ? idx[s + t0 + u] : 0;
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+      if (t0 + u < n) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          bw[u][c] = w[3 * (int64_t)qq[u] + c];
+          bu[u][c] = vel[3 * (int64_t)qq[u] + c];
+        }
+      }
+  };
+  // pass 1, the cell below the tile (direction hd)
+  if (l < NH) {
+    Acc<R> acc[4];
+    auto add_h = [&](R a, const R uu[3]) {
+      acc[0].add(a);
+      acc[1].add(a * uu[0]);
+      acc[2].add(a * uu[1]);
+      acc[3].add(a * uu[2]);
+    };
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+      if (u < nh) add_h(hw[u], hv[u]);
+    for (int32_t t0 = K; t0 < nh; t0 += K) {
+      R bw[K][3], bu[K][3];
+      load_batch(sh, nh, t0, bw, bu);
+#pragma unroll
+      for (int u = 0; u < K; ++u)
+        if (t0 + u < nh) add_h(hd == 0 ? bw[u][0] : hd == 1 ? bw[u][1] : bw[u][2], bu[u]);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      hs[v][l] = acc[v].s;
+      if constexpr (COMP) hc[v][l] = acc[v].c;
+    }
+  }
   // pass 1, own cell: states after the weight-a sums, all three directions
   {
     Acc<R> acc[3][4];
@@ -844,10 +934,12 @@ __global__ __launch_bounds__(kBlock) void k_deposit_tiles(int IM, int JM, int KM
 #pragma unroll
     for (int u = 0; u < K; ++u)
       if (u < n_own) add_a(cw[u], cv[u]);
-    for (int32_t t = K; t < n_own; ++t) {
-      R wa[3], uu[3];
-      own_particle(s_own + t, wa, uu);
-      add_a(wa, uu);
+    for (int32_t t0 = K; t0 < n_own; t0 += K) {
+      R bw[K][3], bu[K][3];
+      load_batch(s_own, n_own, t0, bw, bu);
+#pragma unroll
+      for (int u = 0; u < K; ++u)
+        if (t0 + u < n_own) add_a(bw[u], bu[u]);
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -856,55 +948,6 @@ __global__ __launch_bounds__(kBlock) void k_deposit_tiles(int IM, int JM, int KM
         us[d][v][l] = acc[d][v].s;
         if constexpr (COMP) uc[d][v][l] = acc[d][v].c;
       }
-  }
-  // pass 1, the cells below the tile (one direction each)
-  if (l < NH) {
-    int hi, hj, hk, d;
-    if (l < TY * TZ) {
-      d = 0, hi = bi * TX - 1, hj = bj * TY + l % TY, hk = bk * TZ + l / TY;
-    } else if (l < TY * TZ + TX * TZ) {
-      const int m = l - TY * TZ;
-      d = 1, hi = bi * TX + m % TX, hj = bj * TY - 1, hk = bk * TZ + m / TX;
-    } else {
-      const int m = l - TY * TZ - TX * TZ;
-      d = 2, hi = bi * TX + m % TX, hj = bj * TY + m / TX, hk = bk * TZ - 1;
-    }
-    Acc<R> acc[4];
-    if (hi >= 0 && hj >= 0 && hk >= 0 && hi < IM && hj < JM && hk < KM) {
-      const int32_t key = cell_key(tm, hi, hj, hk);
-      const int32_t sh = kstart[key], nh = kstart[key + 1] - sh;
-      R hw[K], hv[K][3];
-#pragma unroll
-      for (int u = 0; u < K; ++u)
-        if (u < nh) {
-          const int64_t q = idx[sh + u];
-          hw[u] = w[3 * q + d];
-          hv[u][0] = vel[3 * q];
-          hv[u][1] = vel[3 * q + 1];
-          hv[u][2] = vel[3 * q + 2];
-        }
-#pragma unroll
-      for (int u = 0; u < K; ++u)
-        if (u < nh) {
-          acc[0].add(hw[u]);
-          acc[1].add(hw[u] * hv[u][0]);
-          acc[2].add(hw[u] * hv[u][1]);
-          acc[3].add(hw[u] * hv[u][2]);
-        }
-      for (int32_t t = sh + K; t < sh + nh; ++t) {
-        const int64_t q = idx[t];
-        const R a = w[3 * q + d];
-        acc[0].add(a);
-        acc[1].add(a * vel[3 * q]);
-        acc[2].add(a * vel[3 * q + 1]);
-        acc[3].add(a * vel[3 * q + 2]);
-      }
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      hs[v][l] = acc[v].s;
-      if constexpr (COMP) hc[v][l] = acc[v].c;
-    }
   }
   __syncthreads();
   if (i > IM || j > JM || k > KM) return;
@@ -935,10 +978,12 @@ __global__ __launch_bounds__(kBlock) void k_deposit_tiles(int IM, int JM, int KM
 #pragma unroll
   for (int u = 0; u < K; ++u)
     if (u < n_own) add_1ma(cw[u], cv[u]);
-  for (int32_t t = K; t < n_own; ++t) {
-    R wa[3], uu[3];
-    own_particle(s_own + t, wa, uu);
-    add_1ma(wa, uu);
+  for (int32_t t0 = K; t0 < n_own; t0 += K) {
+    R bw[K][3], bu[K][3];
+    load_batch(s_own, n_own, t0, bw, bu);
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+      if (t0 + u < n_own) add_1ma(bw[u], bu[u]);
   }
   R *out[3] = {fi + 4 * (((int64_t)k * JM + j) * (IM + 1) + i), fj + 4 * (((int64_t)k * (JM + 1) + j) * IM + i),
                fk + 4 * (((int64_t)k * JM + j) * IM + i)};
@@ -984,29 +1029,58 @@ __global__ __launch_bounds__(kBlock) void k_count_from_keys(int IM, int JM, int 
 // call (GATHER_STEP) weights and statuses are not moved: the next step
 // recomputes the weights and every particle is ACTIVE_AND_LOCATED.  At the
 // end of a call (GATHER_LAST) weights move and statuses are set to
-// ACTIVE_AND_LOCATED.
+// ACTIVE_AND_LOCATED.  Each thread moves kGatherPer particles, kBlock apart:
+// all their loads are issued before the first store.
 enum GatherMode : int { GATHER_ALL = 0, GATHER_STEP = 1, GATHER_LAST = 2 };
+constexpr int kGatherPer = 4;
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_gather_swarm(int64_t m, int mode, const int32_t *__restrict__ idx,
-                                                         const double *pos, const R *vel, const R *w,
-                                                         const int32_t *cell, const int32_t *status,
-                                                         const int64_t *pid, const R *P, double *pos2, R *vel2,
-                                                         R *w2, int32_t *cell2, int32_t *status2, int64_t *pid2,
-                                                         R *P2) {
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= m) return;
-  int64_t p = idx[q];
+                                                         const double *__restrict__ pos, const R *__restrict__ vel,
+                                                         const R *__restrict__ w, const int32_t *__restrict__ cell,
+                                                         const int32_t *__restrict__ status,
+                                                         const int64_t *__restrict__ pid, const R *__restrict__ P,
+                                                         double *__restrict__ pos2, R *__restrict__ vel2,
+                                                         R *__restrict__ w2, int32_t *__restrict__ cell2,
+                                                         int32_t *__restrict__ status2, int64_t *__restrict__ pid2,
+                                                         R *__restrict__ P2) {
+  constexpr int NP = kGatherPer;
+  const int64_t q0 = (int64_t)blockIdx.x * (kBlock * NP) + threadIdx.x;
+  int64_t p[NP];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    pos2[3 * q + c] = pos[3 * p + c];
-    vel2[3 * q + c] = vel[3 * p + c];
-    if (mode != GATHER_STEP) w2[3 * q + c] = w[3 * p + c];
+  for (int u = 0; u < NP; ++u) p[u] = q0 + u * kBlock < m ? idx[q0 + u * kBlock] : -1;
+  double x[NP][3];
+  R v[NP][3], ww[NP][3], pp[NP];
+  int32_t ce[NP], st[NP];
+  int64_t id[NP];
+#pragma unroll
+  for (int u = 0; u < NP; ++u) {
+    if (p[u] < 0) continue;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      x[u][c] = pos[3 * p[u] + c];
+      v[u][c] = vel[3 * p[u] + c];
+      ww[u][c] = mode != GATHER_STEP ? w[3 * p[u] + c] : (R)0;
+    }
+    ce[u] = cell[p[u]];
+    st[u] = mode == GATHER_ALL ? status[p[u]] : ST_LOCATED;
+    id[u] = pid[p[u]];
+    pp[u] = P[p[u]];
   }
-  cell2[q] = cell[p];
-  if (mode == GATHER_ALL) status2[q] = status[p];
-  if (mode == GATHER_LAST) status2[q] = ST_LOCATED;
-  pid2[q] = pid[p];
-  P2[q] = P[p];
+#pragma unroll
+  for (int u = 0; u < NP; ++u) {
+    if (p[u] < 0) continue;
+    const int64_t q = q0 + u * kBlock;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      pos2[3 * q + c] = x[u][c];
+      vel2[3 * q + c] = v[u][c];
+      if (mode != GATHER_STEP) w2[3 * q + c] = ww[u][c];
+    }
+    cell2[q] = ce[u];
+    if (mode != GATHER_STEP) status2[q] = st[u];
+    pid2[q] = id[u];
+    P2[q] = pp[u];
+  }
 }
 
 }  // namespace picgpu

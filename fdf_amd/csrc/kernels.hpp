@@ -222,18 +222,21 @@ __global__ __launch_bounds__(kBlock) void k_locate(LocateArgs<R> a) {
 // ---------------------------------------------------------------------------
 // Centre -> corner (interpolation.c:598-680): node (i,j,k) = mean of the
 // valid cells (i-1..i, j-1..j, k-1..k), summed dk, dj, di in that nesting.
-// ucat is compact [KM][JM][IM][3].
+// ucat is compact [KM][JM][IM][3].  nx decodes node numbers (IM+1, JM+1 and
+// their reciprocals) without integer division.
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_center_to_corner(int IM, int JM, int KM, const R *__restrict__ ucat,
-                                                             R *__restrict__ corner) {
-  const int64_t nnode = (int64_t)(IM + 1) * (JM + 1) * (KM + 1);
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_center_to_corner(int IM, int JM, int KM, CellIdx nx,
+                                                             const R *__restrict__ ucat, R *__restrict__ corner) {
+  const int64_t plane = (int64_t)(IM + 1) * (JM + 1);
+  const int64_t nnode = plane * (KM + 1);
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nnode) return;
-  int i = (int)(q % (IM + 1));
-  int64_t r = q / (IM + 1);
-  int j = (int)(r % (JM + 1));
-  int k = (int)(r / (JM + 1));
+  // the decoding of unpack_cell, exact for q < 2^50
+  const int k = (int)(((double)q + 0.5) * nx.rIJ);
+  const int64_t r = q - (int64_t)k * plane;
+  const int j = (int)(((double)r + 0.5) * nx.rI);
+  const int i = (int)(r - (int64_t)j * (IM + 1));
   R s0 = 0, s1 = 0, s2 = 0;
   int cnt = 0;
 #pragma unroll
@@ -469,11 +472,12 @@ struct Acc {
   R s = 0, c = 0;
   __device__ __forceinline__ void add(R v) {
     if constexpr (sizeof(R) == 4) {
-      R t = s + v;
-      if (fabsf(s) >= fabsf(v))
-        c += (s - t) + v;
-      else
-        c += (v - t) + s;
+      // Neumaier: c collects the rounding error of every s + v, computed
+      // exactly and without a branch by Knuth's TwoSum (the same value the
+      // |s| >= |v| ordered form gives)
+      const R t = s + v;
+      const R bv = t - s;
+      c += (s - (t - bv)) + (v - bv);
       s = t;
     } else {
       s += v;

@@ -646,7 +646,9 @@ static int do_center_to_corner(picgpu_ctx *c) {
   }
   return dispatch_real(c, [&](auto *tag) -> int {
     using R = std::remove_pointer_t<decltype(tag)>;
-    PG_LAUNCH(c, "center_to_corner", nblocks(c->nnode), kBlock, k_center_to_corner<R>, c->IM, c->JM, c->KM,
+    const CellIdx nx{c->IM + 1, c->JM + 1, 1.0 / (double)(c->IM + 1),
+                     1.0 / ((double)(c->IM + 1) * (double)(c->JM + 1))};
+    PG_LAUNCH(c, "center_to_corner", nblocks(c->nnode), kBlock, k_center_to_corner<R>, c->IM, c->JM, c->KM, nx,
               c->ucat.as<R>(), c->corner.as<R>());
     return PICGPU_OK;
   });
@@ -834,7 +836,8 @@ static int compact_by_key(picgpu_ctx *c, int64_t live, int mode) {
   if (live > 0) {
     int r = dispatch_real(c, [&](auto *tag) -> int {
       using R = std::remove_pointer_t<decltype(tag)>;
-      PG_LAUNCH(c, "gather_swarm", nblocks(live), kBlock, k_gather_swarm<R>, live, mode, idx, c->sw.pos.as<double>(),
+      PG_LAUNCH(c, "gather_swarm", nblocks(live, kBlock * kGatherPer), kBlock, k_gather_swarm<R>, live, mode, idx,
+                c->sw.pos.as<double>(),
                 c->sw.vel.as<R>(), c->sw.w.as<R>(), c->sw.cell.as<int32_t>(), c->sw.status.as<int32_t>(),
                 c->sw.pid.as<int64_t>(), c->sw.P.as<R>(), c->alt.pos.as<double>(), c->alt.vel.as<R>(),
                 c->alt.w.as<R>(), c->alt.cell.as<int32_t>(), c->alt.status.as<int32_t>(), c->alt.pid.as<int64_t>(),
@@ -927,7 +930,8 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, b
   }
   PG_CHECK(scan_exclusive_i32(c, c->kcount.as<int32_t>(), c->kstart[nxt].as<int32_t>(), nkeys + 1));
   if (m > 0)
-    PG_LAUNCH(c, "key_reindex", nblocks(m), kBlock, k_key_reindex, m, idx, c->sw.cell.as<int32_t>(),
+    PG_LAUNCH(c, "key_reindex", nblocks(m, kBlock * kReindexPer), kBlock, k_key_reindex, m, idx,
+              c->sw.cell.as<int32_t>(),
               c->kslot.as<int32_t>(), kbase, CellIdx{c->IM, c->JM, grid_view(c).rI, grid_view(c).rIJ}, tm,
               c->kstart[nxt].as<int32_t>(), c->kidx[nxt].as<int32_t>());
   if (m > 0 && (flags & (PICGPU_STEP_DEPOSIT_CELLS | PICGPU_STEP_DEPOSIT_FACES)))

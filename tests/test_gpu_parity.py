@@ -300,6 +300,53 @@ def test_full_steps_against_oracle(swarm_cls, oracle, case, precision, path):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_crowded_tiles_with_fast_particles(swarm_cls, oracle, precision):
+    """Tiles of 8x8x4 cells holding thousands of particles, more than a
+    thousand of which change cell in a step: the fused step's lanes run many
+    particles each and most walks leave their cell, and its staged results
+    overflow to direct writes.  FP64: bit-exact with the oracle; both
+    precisions: identical to the stage-by-stage path."""
+    _say("crowded_tiles", precision)
+    IM, JM, KM = 16, 16, 8
+    coor = W.perturbed_cube(IM, JM, KM, seed=31, amp=0.15)
+    ucat = W.ucat_random_fourier(coor, IM, JM, KM, seed=32)
+    n = 40000
+    pos = 0.05 + 0.9 * np.random.default_rng(33).random((n, 3))
+    d = {"IM": IM, "JM": JM, "KM": KM, "coor": coor, "ucat": ucat, "pos": pos,
+         "dt": 1.5 / (np.abs(ucat).max() * max(IM, JM, KM))}  # up to ~1.5 cells a step
+    nsteps = 3
+    # by construction: per starting tile, more than 600 particles change cell
+    # in the first step (each with a walk that leaves its cell)
+    s1, _ = _run_oracle_steps(oracle, d, 1, True)
+    c0, _, _, _ = oracle.locate(IM, JM, KM, coor, pos, status=np.full(n, 4, np.int64))
+    start = c0[s1["pid"]]  # (i, j, k)
+    moved = (start != s1["cell"]).any(axis=1)
+    tile = (start[:, 2] // 4) * 4 + (start[:, 1] // 8) * 2 + start[:, 0] // 8  # 2 x 2 x 2 tiles
+    _say("moved per tile", np.bincount(tile[moved], minlength=8))
+    assert np.bincount(tile[moved], minlength=8).max() > 600
+    s, _ = _run_oracle_steps(oracle, d, nsteps, True)
+    outs = []
+    for staged in (False, True):
+        with swarm_cls(precision=precision) as g:
+            g.upload_grid(IM, JM, KM, coor)
+            g.upload_ucat(ucat)
+            g.upload_particles(pos)
+            g.LocateAllParticlesInGrid()
+            g.InterpolateAllFieldsToSwarm()
+            g.AdvanceSimulation(d["dt"], nsteps, rk2=True, deposit_cells=True, deposit_faces=True, staged=staged)
+            outs.append((_by_pid(g.download_particles()), [g.field(f) for f in (L.FIELD_IFACE, L.FIELD_JFACE,
+                                                                                 L.FIELD_KFACE)]))
+    (a, fa), (b, fb) = outs
+    for k in ("pid", "cell", "pos", "vel", "weight", "status"):
+        assert np.array_equal(a[k], b[k]), k
+    for x, y in zip(fa, fb):
+        assert np.array_equal(x, y)
+    if precision == "fp64":
+        for k in ("pid", "cell", "pos", "vel", "weight", "status"):
+            assert np.array_equal(a[k], s[k]), k
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_long_fused_run_matches_staged(swarm_cls, precision):
     """20 steps in one call (the fused path compacts the swarm mid-call) give
     the same particles as the stage-by-stage path."""
