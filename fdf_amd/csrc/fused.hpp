@@ -805,10 +805,10 @@ __global__ __launch_bounds__(kBlock) void k_bucket_pid_order(int64_t nkeys, cons
 // the lower neighbour's state and continues with its own particles (weight
 // 1-a).  A cell's first particles are read once, with independent loads, and
 // kept in registers for both passes; the sums are those of the
-// one-accumulator order above.  Neighbouring threads take neighbouring cells,
-// so that each load instruction of a wavefront reads a few contiguous lines.
-// In FP32 the kernel is held to 5 wavefronts per SIMD (at most 96 VGPRs, no
-// spills), which is what its LDS allows; FP64 needs more registers (4 per
+// one-accumulator order above, plain sums (FaceAcc).  Neighbouring threads
+// take neighbouring cells, so that each load instruction of a wavefront reads
+// a few contiguous lines.  In FP32 the kernel is held to 5 wavefronts per
+// SIMD (at most 96 VGPRs; 6 would spill); FP64 needs more registers (4 per
 // SIMD).
 template <typename R>
 __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 5 : 1) void k_deposit_tiles(
@@ -817,10 +817,9 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 5 : 1) void k_deposit_tile
     R *__restrict__ fj, R *__restrict__ fk) {
   static_assert(TS == kBlock, "one thread per cell of the tile");
   constexpr int NH = TY * TZ + TX * TZ + TX * TY;  // cells below the tile's i-, j- and k-faces
-  constexpr bool COMP = sizeof(R) == 4;             // FP32 sums carry a compensation term
-  __shared__ R us[3][4][TS];                        // own cells: accumulator sum ...
-  __shared__ R uc[COMP ? 3 : 1][4][COMP ? TS : 1];  // ... and compensation
-  __shared__ R hs[4][NH], hc[4][COMP ? NH : 1];     // cells below the tile
+  constexpr bool F32 = sizeof(R) == 4;
+  __shared__ R us[3][4][TS];  // own cells: accumulator states
+  __shared__ R hs[4][NH];     // cells below the tile
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   const int bi = b % xm.NTI, bj = (b / xm.NTI) % xm.NTJ, bk = b / (xm.NTI * xm.NTJ);
   const int l = threadIdx.x;
@@ -832,7 +831,7 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 5 : 1) void k_deposit_tile
   // loads (bucket bounds, indices, particle data) shared by both cells, and
   // the own ones kept in registers for both passes; particles beyond the
   // first K are read in batches of K.
-  constexpr int K = COMP ? 4 : 3;  // FP64: fewer, to stay at 4 wavefronts per SIMD
+  constexpr int K = F32 ? 4 : 3;  // FP64: fewer, to stay at 4 wavefronts per SIMD
   // the cell below the tile's lower face that thread l < NH sums, direction hd
   int hi = -1, hj = -1, hk = -1, hd = 0;
   if (l < TY * TZ) {
@@ -895,7 +894,7 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 5 : 1) void k_deposit_tile
   };
   // pass 1, the cell below the tile (direction hd)
   if (l < NH) {
-    Acc<R> acc[4];
+    FaceAcc<R> acc[4];
     auto add_h = [&](R a, const R uu[3]) {
       acc[0].add(a);
       acc[1].add(a * uu[0]);
@@ -913,14 +912,11 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 5 : 1) void k_deposit_tile
         if (t0 + u < nh) add_h(hd == 0 ? bw[u][0] : hd == 1 ? bw[u][1] : bw[u][2], bu[u]);
     }
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      hs[v][l] = acc[v].s;
-      if constexpr (COMP) hc[v][l] = acc[v].c;
-    }
+    for (int v = 0; v < 4; ++v) hs[v][l] = acc[v].s;
   }
   // pass 1, own cell: states after the weight-a sums, all three directions
   {
-    Acc<R> acc[3][4];
+    FaceAcc<R> acc[3][4];
     auto add_a = [&](const R wa[3], const R uu[3]) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -944,26 +940,20 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 5 : 1) void k_deposit_tile
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        us[d][v][l] = acc[d][v].s;
-        if constexpr (COMP) uc[d][v][l] = acc[d][v].c;
-      }
+      for (int v = 0; v < 4; ++v) us[d][v][l] = acc[d][v].s;
   }
   __syncthreads();
   if (i > IM || j > JM || k > KM) return;
   // pass 2: lower faces, from the lower neighbour's state through the own particles
   const bool has[3] = {j < JM && k < KM, i < IM && k < KM, i < IM && j < JM};
-  Acc<R> acc[3][4];
+  FaceAcc<R> acc[3][4];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const bool inside = d == 0 ? li > 0 : d == 1 ? lj > 0 : lk > 0;
     const int nl = l - (d == 0 ? 1 : d == 1 ? TX : TX * TY);  // lower neighbour in the tile
     const int h = d == 0 ? lj + TY * lk : d == 1 ? TY * TZ + li + TX * lk : TY * TZ + TX * TZ + li + TX * lj;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      acc[d][v].s = inside ? us[d][v][nl] : hs[v][h];
-      if constexpr (COMP) acc[d][v].c = inside ? uc[d][v][nl] : hc[v][h];
-    }
+    for (int v = 0; v < 4; ++v) acc[d][v].s = inside ? us[d][v][nl] : hs[v][h];
   }
   auto add_1ma = [&](const R wa[3], const R uu[3]) {
 #pragma unroll
