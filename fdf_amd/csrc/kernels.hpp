@@ -222,45 +222,95 @@ __global__ __launch_bounds__(kBlock) void k_locate(LocateArgs<R> a) {
 // ---------------------------------------------------------------------------
 // Centre -> corner (interpolation.c:598-680): node (i,j,k) = mean of the
 // valid cells (i-1..i, j-1..j, k-1..k), summed dk, dj, di in that nesting.
-// ucat is compact [KM][JM][IM][3].  nx decodes node numbers (IM+1, JM+1 and
-// their reciprocals) without integer division.
+// ucat is compact [KM][JM][IM][3].
+//
+// A thread takes the node column (i, j) over kC2CChunk consecutive k (grid:
+// x over columns, y over chunks of k).  The four cells of plane k that node k
+// sums last are the four that node k+1 sums first, so each node loads four
+// cells, not eight; the next plane is requested while a node is summed.
+// Neighbouring threads take neighbouring i, so loads and stores are
+// contiguous.
 // ---------------------------------------------------------------------------
+constexpr int kC2CChunk = 32;
+
+// the cells (i-1,j-1), (i,j-1), (i-1,j), (i,j) of plane ck (the (dj, di)
+// order of the sum); v[c] false for a cell outside the grid
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_center_to_corner(int IM, int JM, int KM, CellIdx nx,
-                                                             const R *__restrict__ ucat, R *__restrict__ corner) {
-  const int64_t plane = (int64_t)(IM + 1) * (JM + 1);
-  const int64_t nnode = plane * (KM + 1);
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nnode) return;
-  // the decoding of unpack_cell, exact for q < 2^50
-  const int k = (int)(((double)q + 0.5) * nx.rIJ);
-  const int64_t r = q - (int64_t)k * plane;
-  const int j = (int)(((double)r + 0.5) * nx.rI);
-  const int i = (int)(r - (int64_t)j * (IM + 1));
-  R s0 = 0, s1 = 0, s2 = 0;
-  int cnt = 0;
+__device__ __forceinline__ void c2c_plane(int IM, int JM, int KM, const R *__restrict__ ucat, int i, int j, int ck,
+                                          R u[4][3], bool v[4]) {
 #pragma unroll
-  for (int dk = -1; dk <= 0; ++dk)
+  for (int c = 0; c < 4; ++c) {
+    const int ci = i - 1 + (c & 1), cj = j - 1 + (c >> 1);
+    v[c] = ci >= 0 && ci < IM && cj >= 0 && cj < JM && ck >= 0 && ck < KM;
+    if (v[c]) {
+      const R *p = ucat + 3 * ((int64_t)ci + (int64_t)IM * (cj + (int64_t)JM * ck));
+      u[c][0] = p[0];
+      u[c][1] = p[1];
+      u[c][2] = p[2];
+    } else {
+      u[c][0] = u[c][1] = u[c][2] = (R)0;
+    }
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_center_to_corner(int IM, int JM, int KM, const R *__restrict__ ucat,
+                                                             R *__restrict__ corner) {
+  const uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nI = (uint32_t)IM + 1u;
+  if (col >= nI * ((uint32_t)JM + 1u)) return;
+  const int i = (int)(col % nI), j = (int)(col / nI);
+  const int64_t plane = (int64_t)nI * (JM + 1);
+  // chunks blockIdx.y, blockIdx.y + gridDim.y, ... (one per block unless the
+  // grid's y limit binds)
+  for (int64_t kc = (int64_t)blockIdx.y * kC2CChunk; kc < (int64_t)KM + 1; kc += (int64_t)gridDim.y * kC2CChunk) {
+  const int k0 = (int)kc;
+  const int k1 = k0 + kC2CChunk < KM + 1 ? k0 + kC2CChunk : KM + 1;
+  R lo[4][3], hi[4][3];
+  bool vlo[4], vhi[4];
+  c2c_plane(IM, JM, KM, ucat, i, j, k0 - 1, lo, vlo);
+  c2c_plane(IM, JM, KM, ucat, i, j, k0, hi, vhi);
+  for (int k = k0; k < k1; ++k) {
+    R nx[4][3];
+    bool vnx[4];
+    if (k + 1 < k1) c2c_plane(IM, JM, KM, ucat, i, j, k + 1, nx, vnx);
+    R s0 = 0, s1 = 0, s2 = 0;
+    int cnt = 0;
 #pragma unroll
-    for (int dj = -1; dj <= 0; ++dj)
-#pragma unroll
-      for (int di = -1; di <= 0; ++di) {
-        int ci = i + di, cj = j + dj, ck = k + dk;
-        if (ci >= 0 && ci < IM && cj >= 0 && cj < JM && ck >= 0 && ck < KM) {
-          const R *u = ucat + 3 * ((int64_t)ci + (int64_t)IM * (cj + (int64_t)JM * ck));
-          s0 += u[0];
-          s1 += u[1];
-          s2 += u[2];
-          cnt++;
-        }
+    for (int c = 0; c < 4; ++c)
+      if (vlo[c]) {
+        s0 += lo[c][0];
+        s1 += lo[c][1];
+        s2 += lo[c][2];
+        cnt++;
       }
-  R *o = corner + 3 * q;
-  if (cnt > 0) {
-    o[0] = s0 / (R)cnt;
-    o[1] = s1 / (R)cnt;
-    o[2] = s2 / (R)cnt;
-  } else {
-    o[0] = o[1] = o[2] = (R)0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (vhi[c]) {
+        s0 += hi[c][0];
+        s1 += hi[c][1];
+        s2 += hi[c][2];
+        cnt++;
+      }
+    R *o = corner + 3 * ((int64_t)k * plane + col);
+    if (cnt > 0) {
+      o[0] = s0 / (R)cnt;
+      o[1] = s1 / (R)cnt;
+      o[2] = s2 / (R)cnt;
+    } else {
+      o[0] = o[1] = o[2] = (R)0;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      vlo[c] = vhi[c];
+      vhi[c] = vnx[c];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        lo[c][d] = hi[c][d];
+        hi[c][d] = nx[c][d];
+      }
+    }
+  }
   }
 }
 

@@ -646,10 +646,11 @@ static int do_center_to_corner(picgpu_ctx *c) {
   }
   return dispatch_real(c, [&](auto *tag) -> int {
     using R = std::remove_pointer_t<decltype(tag)>;
-    const CellIdx nx{c->IM + 1, c->JM + 1, 1.0 / (double)(c->IM + 1),
-                     1.0 / ((double)(c->IM + 1) * (double)(c->JM + 1))};
-    PG_LAUNCH(c, "center_to_corner", nblocks(c->nnode), kBlock, k_center_to_corner<R>, c->IM, c->JM, c->KM, nx,
-              c->ucat.as<R>(), c->corner.as<R>());
+    // node columns (i, j) x chunks of kC2CChunk planes
+    const int64_t nch = (c->KM + 1 + kC2CChunk - 1) / kC2CChunk;
+    const dim3 grid((unsigned)nblocks((int64_t)(c->IM + 1) * (c->JM + 1)), (unsigned)std::min<int64_t>(nch, 65535));
+    PG_LAUNCH(c, "center_to_corner", grid, kBlock, k_center_to_corner<R>, c->IM, c->JM, c->KM, c->ucat.as<R>(),
+              c->corner.as<R>());
     return PICGPU_OK;
   });
 }
