@@ -122,6 +122,7 @@ _SIG = {
     "picgpu_timing_enable": (C.c_int, [_P, C.c_int]),
     "picgpu_timing_query": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(_I64)]),
     "picgpu_timing_reset": (C.c_int, [_P]),
+    "picgpu_timing_select": (C.c_int, [_P, C.c_char_p]),
     "picgpu_default_boundary_config": (None, [C.POINTER(PicgpuBoundaryConfig)]),
     "picgpu_parse_boundary_file": (C.c_int, [C.c_char_p, C.POINTER(PicgpuBoundaryConfig)]),
     "picgpu_compute_metrics": (C.c_int, [_P, C.POINTER(C.c_int)]),

@@ -36,6 +36,7 @@ hipEvent_t take_event(picgpu_ctx *c) {
 void timer_begin(picgpu_ctx *c, const char *name, TimerRec &rec) {
   rec.a = rec.b = nullptr;
   if (!c->timing) return;
+  if (!c->timing_only.empty() && strncmp(name, c->timing_only.c_str(), c->timing_only.size()) != 0) return;
   rec.name = name;
   rec.a = take_event(c);
   rec.b = take_event(c);
@@ -854,10 +855,15 @@ static int compact_by_key(picgpu_ctx *c, int64_t live, int mode) {
   return PICGPU_OK;
 }
 
-static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, bool last) {
+// The key index of the previous fused step stays valid until the swarm
+// changes (every entry point that changes cells or statuses clears keyed), so
+// a call that follows a completed call starts from it: the index is then the
+// identity over compacted particles that are all ACTIVE_AND_LOCATED.
+static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool last) {
   const bool rk2 = (flags & PICGPU_STEP_RK2) != 0;
   PG_CHECK(ensure_planes(c));
-  if (first || !c->keyed) {
+  const bool fresh = !c->keyed;  // index built here: statuses still to be read
+  if (fresh) {
     PG_CHECK(build_key_index(c));
     // a swarm in arbitrary order (fresh upload, stage-wise calls) is moved
     // into key order first, so that the tiles read their particles coalesced
@@ -906,7 +912,7 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool first, b
       a.slot = c->kslot.as<int32_t>();
       a.counters = c->counters.as<unsigned long long>();
       a.walk_hist = hist;
-      a.all_located = first ? 0 : 1;
+      a.all_located = fresh ? 0 : 1;
       a.defer_list = c->defer.as<int32_t>();
       a.defer_count = c->defer.as<int32_t>() + (n > 0 ? n : 1);
       constexpr bool EX = sizeof(R) == sizeof(double);
@@ -1078,7 +1084,7 @@ int picgpu_step(picgpu_ctx *c, double dt, int nsteps, unsigned flags) {
     if (!(flags & PICGPU_STEP_STAGED)) {
       c->stats.particle_steps += (s == 0) ? c->n : c->n_live;
       PG_CHECK(do_center_to_corner(c));
-      PG_CHECK(do_step_fused(c, dt, flags, s == 0, s == nsteps - 1));
+      PG_CHECK(do_step_fused(c, dt, flags, s == nsteps - 1));
       c->n_live = c->stats.located;
       continue;
     }
@@ -1261,6 +1267,13 @@ int picgpu_timing_reset(picgpu_ctx *c) {
   if (!c) return PICGPU_E_ARG;
   PG_CHECK(flush_timers(c));
   c->timing_acc.clear();
+  return PICGPU_OK;
+}
+
+int picgpu_timing_select(picgpu_ctx *c, const char *kernel_prefix) {
+  if (!c) return PICGPU_E_ARG;
+  PG_CHECK(flush_timers(c));
+  c->timing_only = kernel_prefix ? kernel_prefix : "";
   return PICGPU_OK;
 }
 

@@ -175,6 +175,7 @@ struct picgpu_ctx {
 
   // timing
   bool timing = false;
+  std::string timing_only;  // when not empty: only kernels whose name starts with it are timed
   std::vector<picgpu::TimerRec> pending;
   std::vector<hipEvent_t> event_pool;
   std::map<std::string, std::pair<double, int64_t>> timing_acc;

@@ -302,6 +302,10 @@ class PicGpu:
     def timing_reset(self):
         L.check(self.lib.picgpu_timing_reset(self.h))
 
+    def timing_select(self, kernel_prefix: str = ""):
+        """Time only the kernels whose name starts with kernel_prefix ("" = all)."""
+        L.check(self.lib.picgpu_timing_select(self.h, kernel_prefix.encode()))
+
     def timing_query(self, kernel: str):
         ms = C.c_double()
         cnt = C.c_int64()

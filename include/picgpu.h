@@ -182,6 +182,10 @@ int picgpu_walk_stats_enable(picgpu_ctx *ctx, int on);
 int picgpu_timing_enable(picgpu_ctx *ctx, int on);
 int picgpu_timing_query(picgpu_ctx *ctx, const char *kernel, double *total_ms, int64_t *launches);
 int picgpu_timing_reset(picgpu_ctx *ctx);
+/* Times only the kernels whose name starts with kernel_prefix (null or "":
+ * all of them).  Each timed launch records two events on the stream; timing
+ * one kernel keeps the other launches free of them. */
+int picgpu_timing_select(picgpu_ctx *ctx, const char *kernel_prefix);
 
 /* ---- Eulerian field preparation: the stage before the particle loop every step ----
  * Reference entry points each call replaces (one MPI rank):

@@ -347,6 +347,37 @@ def test_crowded_tiles_with_fast_particles(swarm_cls, oracle, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_single_step_calls_match_one_call(swarm_cls, precision):
+    """Ten one-step calls (each continuing from the key index the previous
+    call left) give exactly what one ten-step call gives, with and without
+    deposition in between."""
+    _say("single_step_calls", precision)
+    d = W.config(5, scale=0.02)
+    d["pos"] = d["pos"][:30000]
+    outs = []
+    for split in (False, True):
+        with swarm_cls(precision=precision) as g:
+            g.upload_grid(d["IM"], d["JM"], d["KM"], d["coor"])
+            g.upload_ucat(d["ucat"])
+            g.upload_particles(d["pos"])
+            g.LocateAllParticlesInGrid()
+            g.InterpolateAllFieldsToSwarm()
+            if split:
+                for s in range(10):
+                    g.AdvanceSimulation(d["dt"] * 3, 1, rk2=True, deposit_cells=True,
+                                        deposit_faces=s % 3 == 2 or s == 9)
+            else:
+                g.AdvanceSimulation(d["dt"] * 3, 10, rk2=True, deposit_cells=True, deposit_faces=True)
+            st = g.stats()
+            outs.append((_by_pid(g.download_particles()), g.field(L.FIELD_KFACE), g.field(L.FIELD_P_AVERAGE),
+                         st["located"]))
+    (a, fa, pa, la), (b, fb, pb, lb) = outs
+    for k in ("pid", "cell", "pos", "vel", "weight", "status"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(fa, fb) and np.array_equal(pa, pb) and la == lb
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_long_fused_run_matches_staged(swarm_cls, precision):
     """20 steps in one call (the fused path compacts the swarm mid-call) give
     the same particles as the stage-by-stage path."""
