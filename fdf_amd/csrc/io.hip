@@ -266,6 +266,8 @@ int picgpu_write_output(picgpu_ctx *c, const char *dir, int64_t step) {
     set_error("no grid uploaded");
     return PICGPU_E_STATE;
   }
+  PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(settle_swarm(c));
   const int64_t n = c->n, nn = c->nnode;
   std::vector<double> buf((size_t)std::max<int64_t>(3 * n, 3 * nn));
   PG_CHECK(picgpu_download_particles(c, buf.data(), nullptr, nullptr, nullptr, nullptr, nullptr));
@@ -483,6 +485,8 @@ int picgpu_write_vtk(picgpu_ctx *c, const char *prefix, int64_t step) {
     set_error("no grid uploaded");
     return PICGPU_E_STATE;
   }
+  PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(settle_swarm(c));
   const int64_t nn = c->nnode, n = c->n;
   std::vector<double> coor((size_t)(3 * nn)), f((size_t)(3 * nn));
   PG_HIP(hipMemcpy(coor.data(), c->coor.ptr, sizeof(double) * coor.size(), hipMemcpyDeviceToHost));

@@ -531,6 +531,8 @@ int picgpu_mlp_train(picgpu_ctx *c, int64_t max_pairs, int batch, int iters, flo
     set_error("mlp_train: bad arguments (batch must be a positive multiple of 32)");
     return PICGPU_E_ARG;
   }
+  PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(settle_swarm(c));
   if (!c->have_grid || c->n == 0) {
     set_error("mlp_train: needs a grid and a located swarm");
     return PICGPU_E_STATE;

@@ -181,7 +181,7 @@ static int finish_grid(picgpu_ctx *c) {
   c->have_planes = false;
   c->have_ucat = false;
   c->eul.have_metrics = c->eul.have_bc = false;
-  c->sorted = c->keyed = c->key_ordered = false;
+  c->sorted = c->keyed = c->key_ordered = c->compact_pending = false;
   c->have_pavg = c->have_faces = false;
   return PICGPU_OK;
 }
@@ -296,6 +296,8 @@ int picgpu_upload_grid(picgpu_ctx *c, int64_t IM, int64_t JM, int64_t KM, const 
     set_error("upload_grid: bad arguments");
     return PICGPU_E_ARG;
   }
+  PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(settle_swarm(c));  // the particles stay; the new grid drops their index
   if (IM * JM * KM >= (int64_t)INT32_MAX - 2 || (IM + 1) * (JM + 1) * (KM + 1) >= ((int64_t)1 << 40)) {
     set_error("upload_grid: grid too large for 32-bit packed cell indices");
     return PICGPU_E_UNSUPPORTED;
@@ -316,6 +318,8 @@ int picgpu_upload_grid_device(picgpu_ctx *c, int64_t IM, int64_t JM, int64_t KM,
     set_error("upload_grid_device: bad arguments");
     return PICGPU_E_ARG;
   }
+  PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(settle_swarm(c));  // the particles stay; the new grid drops their index
   if (IM * JM * KM >= (int64_t)INT32_MAX - 2) {
     set_error("upload_grid_device: grid too large for 32-bit packed cell indices");
     return PICGPU_E_UNSUPPORTED;
@@ -360,7 +364,7 @@ int picgpu_init_particles_device(picgpu_ctx *c, int64_t n, const double *d_pos, 
   PG_HIP(hipSetDevice(c->device));
   PG_CHECK(ensure_swarm(c, c->sw, n));
   c->n = n;
-  c->sorted = c->keyed = c->key_ordered = false;
+  c->sorted = c->keyed = c->key_ordered = c->compact_pending = false;
   if (n == 0) return PICGPU_OK;
   int r = dispatch_real(c, [&](auto *tag) -> int {
     using R = std::remove_pointer_t<decltype(tag)>;
@@ -416,7 +420,7 @@ int picgpu_upload_particles(picgpu_ctx *c, int64_t n, const double *pos, const d
   PG_HIP(hipSetDevice(c->device));
   PG_CHECK(ensure_swarm(c, c->sw, n));
   c->n = n;
-  c->sorted = c->keyed = c->key_ordered = false;
+  c->sorted = c->keyed = c->key_ordered = c->compact_pending = false;
   if (n == 0) return PICGPU_OK;
   PG_HIP(hipMemcpy(c->sw.pos.ptr, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice));
   PG_HIP(hipMemcpy(c->sw.pid.ptr, pid, sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice));
@@ -446,15 +450,16 @@ int picgpu_upload_particles(picgpu_ctx *c, int64_t n, const double *pos, const d
 
 int picgpu_num_particles(picgpu_ctx *c, int64_t *n) {
   if (!c || !n) return PICGPU_E_ARG;
-  *n = c->n;
+  *n = live_particles(c);
   return PICGPU_OK;
 }
 
 int picgpu_download_particles(picgpu_ctx *c, double *pos, double *vel, int64_t *cell, double *weight,
                               int64_t *status, int64_t *pid) {
   if (!c) return PICGPU_E_ARG;
-  const int64_t n = c->n;
   PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(settle_swarm(c));
+  const int64_t n = c->n;
   PG_HIP(hipStreamSynchronize(c->stream));
   if (n == 0) return PICGPU_OK;
   if (pos) PG_HIP(hipMemcpy(pos, c->sw.pos.ptr, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost));
@@ -494,6 +499,8 @@ int picgpu_download_particles(picgpu_ctx *c, double *pos, double *vel, int64_t *
 
 int picgpu_download_walk_steps(picgpu_ctx *c, int32_t *steps) {
   if (!c || !steps) return PICGPU_E_ARG;
+  PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(settle_swarm(c));
   if (!c->opt.collect_walk_stats || !c->walk_steps.ptr || c->walk_steps.bytes < sizeof(int32_t) * (size_t)c->n) {
     set_error("walk statistics not collected (collect_walk_stats=0 or no locate yet)");
     return PICGPU_E_STATE;
@@ -509,12 +516,13 @@ int picgpu_download_walk_steps(picgpu_ctx *c, int32_t *steps) {
 // stage implementations
 // ---------------------------------------------------------------------------
 static int do_reset_advect(picgpu_ctx *c, double dt, int reset, int advect) {
+  PG_CHECK(settle_swarm(c));
   if (c->n == 0) return PICGPU_OK;
   return dispatch_real(c, [&](auto *tag) -> int {
     using R = std::remove_pointer_t<decltype(tag)>;
     PG_LAUNCH(c, "reset_advect", nblocks(c->n), kBlock, k_reset_advect<R>, c->n, c->sw.pos.as<double>(),
               c->sw.vel.as<R>(), c->sw.status.as<int32_t>(), dt, reset, advect);
-    c->sorted = c->keyed = c->key_ordered = false;
+    c->sorted = c->keyed = c->key_ordered = c->compact_pending = false;
     return PICGPU_OK;
   });
 }
@@ -539,6 +547,7 @@ static int ensure_planes(picgpu_ctx *c) {
 
 static int do_locate(picgpu_ctx *c, int skip_lost) {
   PG_CHECK(need_grid(c));
+  PG_CHECK(settle_swarm(c));
   PG_CHECK(ensure_planes(c));
   PG_CHECK(counters_reset(c));
   const int64_t n = c->n;
@@ -599,12 +608,13 @@ static int do_locate(picgpu_ctx *c, int skip_lost) {
   c->stats.located = (int64_t)h[C_LOCATED];
   c->stats.lost = (int64_t)h[C_LOST];
   if (n > 0) PG_CHECK(hist_read(c));
-  c->sorted = c->keyed = c->key_ordered = false;
+  c->sorted = c->keyed = c->key_ordered = c->compact_pending = false;
   return PICGPU_OK;
 }
 
 // Stable compaction of the swarm keeping particles by the flags.
 static int do_remove(picgpu_ctx *c, int check_lost, int check_box, int64_t *removed_lost, int64_t *removed_oob) {
+  PG_CHECK(settle_swarm(c));
   const int64_t n = c->n;
   if (removed_lost) *removed_lost = 0;
   if (removed_oob) *removed_oob = 0;
@@ -636,7 +646,7 @@ static int do_remove(picgpu_ctx *c, int check_lost, int check_box, int64_t *remo
   PG_CHECK(r);
   swap_swarm(c);
   c->n = total;
-  c->sorted = c->keyed = c->key_ordered = false;
+  c->sorted = c->keyed = c->key_ordered = c->compact_pending = false;
   return PICGPU_OK;
 }
 
@@ -657,6 +667,7 @@ static int do_center_to_corner(picgpu_ctx *c) {
 }
 
 static int do_interp(picgpu_ctx *c) {
+  PG_CHECK(settle_swarm(c));
   if (c->n == 0) return PICGPU_OK;
   return dispatch_real(c, [&](auto *tag) -> int {
     using R = std::remove_pointer_t<decltype(tag)>;
@@ -668,6 +679,7 @@ static int do_interp(picgpu_ctx *c) {
 
 // Counting sort by cell; particles without a cell go to a last bucket.
 static int do_sort(picgpu_ctx *c) {
+  PG_CHECK(settle_swarm(c));
   const int64_t n = c->n;
   const int64_t nb = c->ncell + 1;  // + unlocated bucket
   PG_CHECK(c->cell_count.ensure(sizeof(int32_t) * (size_t)(nb + 1)));
@@ -855,10 +867,24 @@ static int compact_by_key(picgpu_ctx *c, int64_t live, int mode) {
   return PICGPU_OK;
 }
 
+int picgpu::settle_swarm(picgpu_ctx *c) {
+  if (!c->compact_pending) return PICGPU_OK;
+  c->compact_pending = false;
+  PG_CHECK(compact_by_key(c, c->n_live, GATHER_LAST));
+  c->steps_since_compact = 0;
+  return PICGPU_OK;
+}
+
+int64_t picgpu::live_particles(const picgpu_ctx *c) { return c->compact_pending ? c->n_live : c->n; }
+
 // The key index of the previous fused step stays valid until the swarm
 // changes (every entry point that changes cells or statuses clears keyed), so
-// a call that follows a completed call starts from it: the index is then the
-// identity over compacted particles that are all ACTIVE_AND_LOCATED.
+// a call that follows a step call starts from it.  Every particle of that
+// index is ACTIVE_AND_LOCATED; if the previous call ended between
+// compactions, the arrays still hold its removed particles (compact_pending)
+// and the compaction schedule (every kCompactEvery steps) carries on.
+// last: the call's last step, which compacts with weights and statuses if
+// the schedule falls on it.
 static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool last) {
   const bool rk2 = (flags & PICGPU_STEP_RK2) != 0;
   PG_CHECK(ensure_planes(c));
@@ -873,6 +899,7 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool last) {
   const int64_t nt = ntiles(tm), nkeys = nt * TS;
   const int64_t n = c->n, m = c->n_live;  // m: entries of the current index
   const int cur = c->kcur, nxt = cur ^ 1;
+  c->stats.particle_steps += m;
   PG_CHECK(counters_reset(c));
   PG_CHECK(c->kstart[nxt].ensure(sizeof(int32_t) * (size_t)(nkeys + 2)));
   PG_CHECK(c->kidx[nxt].ensure(sizeof(int32_t) * (size_t)(n > 0 ? n : 1)));
@@ -947,6 +974,9 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool last) {
   unsigned long long h[C_NUM];
   PG_CHECK(counters_read(c, h));
   if (h[C_ERROR]) {
+    // drop the particles earlier steps removed (through the index this step
+    // started from) before the index is given up
+    PG_CHECK(settle_swarm(c));
     c->keyed = false;
     set_error("degenerate cell face (|n| < 1e-12) met during the walk");
     return PICGPU_E_DEGENERATE;
@@ -984,10 +1014,13 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool last) {
   });
   PG_CHECK(r);
   c->steps_since_compact++;
-  if (last || c->steps_since_compact >= kCompactEvery) {
-    // every survivor of a fused step is located
+  c->compact_pending = true;
+  if (c->steps_since_compact >= kCompactEvery) {
+    // every survivor of a fused step is located; at the call's last step the
+    // weights and statuses move too
     PG_CHECK(compact_by_key(c, live, last ? GATHER_LAST : GATHER_STEP));
     c->steps_since_compact = 0;
+    c->compact_pending = false;
   }
   c->sorted = false;  // key order, not cell order
   return PICGPU_OK;
@@ -1076,13 +1109,13 @@ int picgpu_step(picgpu_ctx *c, double dt, int nsteps, unsigned flags) {
     set_error("ucat not uploaded");
     return PICGPU_E_STATE;
   }
-  c->steps_since_compact = 0;
+  if (flags & PICGPU_STEP_STAGED) PG_CHECK(settle_swarm(c));
+  if (!c->compact_pending) c->steps_since_compact = 0;
   const bool fields = (flags & PICGPU_STEP_FIELDS) != 0;
   if (fields) PG_CHECK(eulerian_fields_begin(c));
   for (int s = 0; s < nsteps; ++s) {
     if (fields) PG_CHECK(eulerian_fields_async(c));
     if (!(flags & PICGPU_STEP_STAGED)) {
-      c->stats.particle_steps += (s == 0) ? c->n : c->n_live;
       PG_CHECK(do_center_to_corner(c));
       PG_CHECK(do_step_fused(c, dt, flags, s == nsteps - 1));
       c->n_live = c->stats.located;
@@ -1237,7 +1270,7 @@ int picgpu_download_grid_field(picgpu_ctx *c, int which, double *out) {
 int picgpu_get_stats(picgpu_ctx *c, picgpu_stats *st) {
   if (!c || !st) return PICGPU_E_ARG;
   *st = c->stats;
-  st->n_particles = c->n;
+  st->n_particles = live_particles(c);
   return PICGPU_OK;
 }
 

@@ -156,6 +156,10 @@ struct picgpu_ctx {
   int kcur = 0;              // index in use
   bool keyed = false;        // kidx[kcur] / kstart[kcur] describe the current swarm
   bool key_ordered = false;  // the swarm arrays are (nearly) in key order: compacted since the last change
+  // a step call ended between compactions: kidx[kcur] lists the n_live live
+  // particles and the arrays (length n) still hold removed ones; settle_swarm
+  // compacts them before anything else reads or changes the arrays
+  bool compact_pending = false;
   int64_t kbegin_unloc = 0, kend_unloc = 0;  // range of bucket nkeys in kidx[kcur]
 
   // per-cell / per-face outputs
@@ -199,6 +203,12 @@ int flush_timers(picgpu_ctx *c);
 
 // device exclusive scan of int32 (n entries -> out[0..n], out[n] = total)
 int scan_exclusive_i32(picgpu_ctx *c, const int32_t *in, int32_t *out, int64_t n);
+
+// Compacts a swarm that a step call left uncompacted (compact_pending); every
+// entry point that reads or changes the particle arrays calls it first.
+int settle_swarm(picgpu_ctx *c);
+// particles of the swarm: n_live while a compaction is pending, else n
+int64_t live_particles(const picgpu_ctx *c);
 
 // Eulerian field preparation (eulerian.hip)
 void eulerian_release(picgpu_ctx *c);

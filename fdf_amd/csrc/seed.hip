@@ -174,6 +174,7 @@ int picgpu_reinit_on_inlet(picgpu_ctx *c, int inlet_face, int64_t seed) {
     return PICGPU_E_STATE;
   }
   PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(settle_swarm(c));
   const int64_t n = c->n;
   DevBuf tmp;
   int r = seed_positions(c, n, 0, inlet_face, seed, tmp);
