@@ -80,6 +80,52 @@ def test_write_output_then_restart(swarm_cls, oracle, tmp_path, precision):
     np.testing.assert_array_equal(r2["status"][o], st)
 
 
+def test_entry_points_after_single_step_calls(swarm_cls, tmp_path):
+    """Step calls that end between compactions leave the removed particles in
+    the arrays; output files, stage-wise calls and a grid upload must see only
+    the live particles."""
+    _say("entry points after single-step calls")
+    IM, JM, KM = 16, 16, 40
+    coor = W.tube_grid(IM, JM, KM)
+    pos = W.particles_uniform_tube(20000, 92)
+    with swarm_cls(precision="fp64") as g:
+        g.upload_grid(IM, JM, KM, coor)
+        g.compute_metrics()
+        g.BoundarySystem_Create(channel(4, "constant_velocity", 1.0))
+        g.SetEulerianFields(0, 0, 1, (0.0, 0.0, 1.0))
+        g.upload_particles(pos)
+        g.LocateAllParticlesInGrid()
+        g.InterpolateAllFieldsToSwarm()
+        for _ in range(3):  # one call per step, no compaction yet
+            g.AdvanceSimulation(0.08, 1, rk2=True, deposit_cells=True)
+        n = g.n
+        assert n == g.stats()["n_particles"] < 20000  # some left through the open end
+        g.WriteOutput(3, str(tmp_path))
+        d = g.download_particles()
+        assert len(d["pid"]) == n and (d["status"] == L.ACTIVE_AND_LOCATED).all()
+        assert np.array_equal(_f(tmp_path, "position", 3), d["pos"].ravel())
+        # the step interpolated at the final positions: the stage-wise call
+        # (which compacts first) gives the same velocities
+        g.AdvanceSimulation(0.08, 2, rk2=True, deposit_cells=True)
+        n2 = g.n
+        g.InterpolateAllFieldsToSwarm()
+        d2 = g.download_particles()
+        assert len(d2["pid"]) == n2
+        g.AdvanceSimulation(0.08, 1, rk2=True, deposit_cells=True)
+        d3 = g.download_particles()
+        g.InterpolateAllFieldsToSwarm()
+        assert np.array_equal(g.download_particles()["vel"], d3["vel"])
+        # a grid upload keeps the live particles only
+        g.AdvanceSimulation(0.08, 1, rk2=True, deposit_cells=True)
+        n3 = g.n
+        g.upload_grid(IM, JM, KM, coor)
+        assert g.n == n3
+        g.compute_metrics()
+        g.upload_ucat(np.zeros(((IM + 1) * (JM + 1) * (KM + 1), 3)))
+        g.LocateAllParticlesInGrid()
+        assert (g.download_particles()["status"] == L.ACTIVE_AND_LOCATED).all()
+
+
 def test_restart_errors(swarm_cls, tmp_path):
     _say("restart errors")
     IM, JM, KM = 4, 4, 4
