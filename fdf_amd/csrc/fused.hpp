@@ -807,17 +807,15 @@ __global__ __launch_bounds__(kBlock) void k_bucket_pid_order(int64_t nkeys, cons
 // kept in registers for both passes; the sums are those of the
 // one-accumulator order above, plain sums (FaceAcc).  Neighbouring threads
 // take neighbouring cells, so that each load instruction of a wavefront reads
-// a few contiguous lines.  In FP32 the kernel is held to 5 wavefronts per
-// SIMD (at most 96 VGPRs; 6 would spill); FP64 needs more registers (4 per
-// SIMD).
+// a few contiguous lines.  In FP32 the kernel runs 6 wavefronts per SIMD (at
+// most 80 VGPRs, no spills); FP64 needs more registers (4 per SIMD).
 template <typename R>
-__global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 5 : 1) void k_deposit_tiles(
+__global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 6 : 1) void k_deposit_tiles(
     int IM, int JM, int KM, TileMap tm, TileMap xm, const int32_t *__restrict__ kstart,
     const int32_t *__restrict__ idx, const R *__restrict__ w, const R *__restrict__ vel, R *__restrict__ fi,
     R *__restrict__ fj, R *__restrict__ fk) {
   static_assert(TS == kBlock, "one thread per cell of the tile");
   constexpr int NH = TY * TZ + TX * TZ + TX * TY;  // cells below the tile's i-, j- and k-faces
-  constexpr bool F32 = sizeof(R) == 4;
   __shared__ R us[3][4][TS];  // own cells: accumulator states
   __shared__ R hs[4][NH];     // cells below the tile
   const int b = xcd_tile(blockIdx.x, gridDim.x);
@@ -831,7 +829,7 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 5 : 1) void k_deposit_tile
   // loads (bucket bounds, indices, particle data) shared by both cells, and
   // the own ones kept in registers for both passes; particles beyond the
   // first K are read in batches of K.
-  constexpr int K = F32 ? 4 : 3;  // FP64: fewer, to stay at 4 wavefronts per SIMD
+  constexpr int K = 3;  // 6 wavefronts per SIMD in FP32 (4 particles: 5; 2: no faster)
   // the cell below the tile's lower face that thread l < NH sums, direction hd
   int hi = -1, hj = -1, hk = -1, hd = 0;
   if (l < TY * TZ) {
