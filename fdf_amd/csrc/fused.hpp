@@ -774,7 +774,7 @@ __global__ __launch_bounds__(kBlock) void k_bucket_pid_order(int64_t nkeys, cons
       int r = 0;
 #pragma unroll
       for (int b = 0; b < 8; ++b) r += (v[b] < v[a] || (v[b] == v[a] && b < a)) ? 1 : 0;
-      if (a < L) idx[s + r] = q[a];
+      if (a < L && r != a) idx[s + r] = q[a];  // entries already in place are not rewritten
     }
     return;
   }
