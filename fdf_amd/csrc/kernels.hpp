@@ -516,7 +516,8 @@ __device__ __forceinline__ int64_t next_by_pid(const int64_t *__restrict__ pid, 
   return best;
 }
 
-// Neumaier-compensated accumulator for the FP32 deposition; plain sum in FP64.
+// Neumaier-compensated accumulator of the P averages in FP32 (a cell's sum of
+// equal P values stays exact at any count); plain sum in FP64.
 template <typename R>
 struct Acc {
   R s = 0, c = 0;
