@@ -134,6 +134,18 @@ struct PlaneSrc {
       return eval_cell_planes<R, EXACT>(planes, g, i, j, k, x, ts, d, fb) ? EV_OK : EV_DEGENERATE;
     }
   }
+  // the same, given the cell's linear index lc = i + IM (j + JM k)
+  template <bool EXACT>
+  __device__ __forceinline__ int eval_lin(int i, int j, int k, int32_t lc, const double x[3], double ts, R d[6],
+                                          unsigned long long &fb) const {
+    if constexpr (DEFER && !EXACT) {
+      return planes_fp32_distances(planes[lc], x, d) ? EV_DEFER : EV_OK;
+    } else if constexpr (EXACT) {
+      return eval_planes_exact(planes[lc], x, d) ? EV_OK : EV_DEGENERATE;
+    } else {
+      return eval_cell_planes<R, EXACT>(planes, g, i, j, k, x, ts, d, fb) ? EV_OK : EV_DEGENERATE;
+    }
+  }
   __device__ __forceinline__ void corners(int i, int j, int k, R C[8][3]) const {
     if constexpr (TILE) {
       if ((unsigned)(i - i0) < (unsigned)(BX - 1) && (unsigned)(j - j0) < (unsigned)(BY - 1) &&
@@ -222,6 +234,57 @@ __device__ __forceinline__ int walk_iter(const Src &src, const double x[3], int 
     return W_LOCATED;
   }
   step_indices<R>(d, ci, cj, ck);
+  return W_CONTINUE;
+}
+
+// walk_iter that also carries the cell's linear index lc = ci + IM (cj + JM
+// ck): a step moves each of ci, cj, ck by at most one (step_indices), so lc
+// follows with adds of 1, IM and IM JM instead of being recomputed with
+// 64-bit multiplies for the record load and the cycle guard.  lc may leave
+// [0, IM JM KM) only together with (ci, cj, ck), which the bounds test
+// rejects before lc is used.
+template <typename R, bool EXACT, class Src>
+__device__ __forceinline__ int walk_iter_lin(const Src &src, const double x[3], int &ci, int &cj, int &ck,
+                                             int32_t &lc, int &steps, CycleGuard &cyc, int max_traversal,
+                                             double thresh_scale, R clamp, R wv[3], unsigned long long &fallbacks) {
+  const GridView &g = src.g;
+  if (steps >= max_traversal) return W_LOST;
+  steps++;
+  if (ci < 0 || ci >= g.IM || cj < 0 || cj >= g.JM || ck < 0 || ck >= g.KM) return W_LOST;
+  if (cyc.revisit(lc)) {
+    steps = max_traversal;
+    return W_LOST;
+  }
+  R d[6];
+  const int ev = src.template eval_lin<EXACT>(ci, cj, ck, lc, x, thresh_scale, d, fallbacks);
+  if (ev == EV_DEGENERATE) return W_DEGENERATE;
+  if (ev == EV_DEFER) return W_DEFER;
+  if (point_position<R>(d) >= 0) {
+    weights_from_distances<R>(d, clamp, wv);
+    return W_LOCATED;
+  }
+  const int32_t sj = g.IM, sk = g.IM * g.JM;
+  if (d[F_FRONT] < (R)0) {
+    ck += 1;
+    lc += sk;
+  } else if (d[F_BACK] < (R)0) {
+    ck -= 1;
+    lc -= sk;
+  }
+  if (d[F_LEFT] < (R)0) {
+    ci -= 1;
+    lc -= 1;
+  } else if (d[F_RIGHT] < (R)0) {
+    ci += 1;
+    lc += 1;
+  }
+  if (d[F_BOTTOM] < (R)0) {
+    cj -= 1;
+    lc -= sj;
+  } else if (d[F_TOP] < (R)0) {
+    cj += 1;
+    lc += sj;
+  }
   return W_CONTINUE;
 }
 
@@ -495,6 +558,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
   int ph = 0;
   double xw[3] = {0, 0, 0};
   int ci = 0, cj = 0, ck = 0, steps = 0, stot = 0;
+  int32_t lc = 0;  // ci + IM (cj + JM ck)
   CycleGuard cyc;
   R wv[3] = {0, 0, 0};
   int32_t fq = -1, fv = 0;  // a slot taken in the global histogram, stored at the next one (or the end)
@@ -505,6 +569,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
     pnn = q + 2 * B < e ? idx[q + 2 * B] : -1;
     ci = cj = ck = 0;
     if (ps.c0 >= 0) unpack_cell(ps.c0, cell_idx(a.g), ci, cj, ck);
+    lc = ps.c0 >= 0 ? ps.c0 : 0;
     steps = stot = 0;
     cyc = CycleGuard{};
 #pragma unroll
@@ -531,7 +596,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
     if (ph >= 3) {
       r = W_LOST;
     } else {
-      r = walk_iter<R, EXACT>(src, xw, ci, cj, ck, steps, cyc, a.max_traversal, a.thresh_scale, a.clamp, wv,
+      r = walk_iter_lin<R, EXACT>(src, xw, ci, cj, ck, lc, steps, cyc, a.max_traversal, a.thresh_scale, a.clamp, wv,
                               tally.fb);
       if (r == W_CONTINUE) continue;
       if (r == W_LOCATED) interp_at<R>(src, ci, cj, ck, wv, ui);
@@ -571,7 +636,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
       } else {
         const bool in = xw[0] >= a.bx0 && xw[0] <= a.bx1 && xw[1] >= a.by0 && xw[1] <= a.by1 && xw[2] >= a.bz0 &&
                         xw[2] <= a.bz1;
-        cell = pack_cell(ci, cj, ck, a.g.IM, a.g.JM);
+        cell = lc;
         if (in) {
           tally.located++;
           kind = OUT_LOC;
