@@ -229,7 +229,7 @@ __device__ __forceinline__ int walk_iter(const Src &src, const double x[3], int 
   const int ev = src.template eval<EXACT>(ci, cj, ck, x, thresh_scale, d, fallbacks);
   if (ev == EV_DEGENERATE) return W_DEGENERATE;
   if (ev == EV_DEFER) return W_DEFER;
-  if (point_position<R>(d) >= 0) {
+  if (point_accepted<R>(d)) {
     weights_from_distances<R>(d, clamp, wv);
     return W_LOCATED;
   }
@@ -259,7 +259,7 @@ __device__ __forceinline__ int walk_iter_lin(const Src &src, const double x[3], 
   const int ev = src.template eval_lin<EXACT>(ci, cj, ck, lc, x, thresh_scale, d, fallbacks);
   if (ev == EV_DEGENERATE) return W_DEGENERATE;
   if (ev == EV_DEFER) return W_DEFER;
-  if (point_position<R>(d) >= 0) {
+  if (point_accepted<R>(d)) {
     weights_from_distances<R>(d, clamp, wv);
     return W_LOCATED;
   }

@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void k_locate(LocateArgs<R> a) {
       if (degenerate) {
         my_err = 1;
         done = true;
-      } else if (point_position<R>(d) >= 0) {
+      } else if (point_accepted<R>(d)) {
         R wv[3];
         weights_from_distances<R>(d, a.clamp, wv);
         a.w[3 * p] = wv[0];

@@ -185,6 +185,17 @@ __device__ __forceinline__ int point_position(const R d[6]) {
   return zeros >= 3 ? 3 : zeros;
 }
 
+// point_position(d) >= 0, the walk's acceptance (walkingsearch.c:1150-1160):
+// inside (no distance below zero) or on a face (some distance exactly zero).
+// Computed as a minimum and six equality tests, without the count.
+template <typename R>
+__device__ __forceinline__ bool point_accepted(const R d[6]) {
+  const R mn = fmin(fmin(fmin(d[0], d[1]), fmin(d[2], d[3])), fmin(d[4], d[5]));
+  const bool on_face = d[0] == (R)0 || d[1] == (R)0 || d[2] == (R)0 || d[3] == (R)0 || d[4] == (R)0 ||
+                       d[5] == (R)0;
+  return !(mn < (R)0) || on_face;
+}
+
 // walkingsearch.c:919-946
 template <typename R>
 __device__ __forceinline__ void step_indices(const R d[6], int &i, int &j, int &k) {
