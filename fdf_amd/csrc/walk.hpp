@@ -170,24 +170,12 @@ __device__ __forceinline__ bool eval_cell_exact(const double V[8][3], const doub
          face_exact<3>(V, cc, p, thr, d) && face_exact<4>(V, cc, p, thr, d) && face_exact<5>(V, cc, p, thr, d);
 }
 
-// walkingsearch.c:407-462
-template <typename R>
-__device__ __forceinline__ int point_position(const R d[6]) {
-  bool inside = true;
-  int zeros = 0;
-#pragma unroll
-  for (int f = 0; f < 6; ++f) {
-    if (d[f] < (R)0) inside = false;
-    if (d[f] == (R)0) zeros++;
-  }
-  if (inside) return 0;
-  if (zeros == 0) return -1;
-  return zeros >= 3 ? 3 : zeros;
-}
-
-// point_position(d) >= 0, the walk's acceptance (walkingsearch.c:1150-1160):
-// inside (no distance below zero) or on a face (some distance exactly zero).
-// Computed as a minimum and six equality tests, without the count.
+// The walk's acceptance.  The reference classifies a point as inside (0: no
+// distance below zero), outside (-1: some below zero, none exactly zero) or
+// on 1..3 faces (the number of zero distances, capped at 3)
+// (walkingsearch.c:407-462) and accepts any classification >= 0
+// (:1150-1160): inside, or some distance exactly zero.  That is a minimum and
+// six equality tests; the count itself is never used.
 template <typename R>
 __device__ __forceinline__ bool point_accepted(const R d[6]) {
   const R mn = fmin(fmin(fmin(d[0], d[1]), fmin(d[2], d[3])), fmin(d[4], d[5]));
