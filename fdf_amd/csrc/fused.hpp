@@ -10,9 +10,10 @@
 //
 // Cells are grouped in 8x8x4 tiles; particles are kept in tile-major cell
 // order (a counting sort by the tile-major cell key after every step).  One
-// workgroup processes the particles of one tile and first stages the
-// tile's node coordinates and corner velocities, with a one-cell halo, in
-// LDS; walks that stay within the halo read LDS only.  Each particle ends by
+// workgroup processes the particles of one tile and first stages the tile's
+// corner velocities, with a one-cell halo, in LDS; interpolations that stay
+// within the halo read LDS only.  Walk evaluations read each cell's
+// precomputed face-plane record from global memory.  Each particle ends by
 // taking a slot in the bucket of its new key, from which the next step's
 // index is built.  Removed particles stay in the arrays, marked ST_REMOVED,
 // until the swarm is compacted.
@@ -872,8 +873,9 @@ __global__ __launch_bounds__(kBlock) void k_bucket_pid_order(int64_t nkeys, cons
 // kept in registers for both passes; the sums are those of the
 // one-accumulator order above, plain sums (FaceAcc).  Neighbouring threads
 // take neighbouring cells, so that each load instruction of a wavefront reads
-// a few contiguous lines.  In FP32 the kernel runs 6 wavefronts per SIMD (at
-// most 80 VGPRs, no spills); FP64 needs more registers (4 per SIMD).
+// a few contiguous lines.  In FP32 the kernel is held to at least 6
+// wavefronts per SIMD (at most 80 VGPRs; it takes 69, so 7 fit, with no
+// spills); FP64 needs more registers (4 per SIMD).
 template <typename R>
 __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 6 : 1) void k_deposit_tiles(
     int IM, int JM, int KM, TileMap tm, TileMap xm, const int32_t *__restrict__ kstart,
