@@ -379,6 +379,33 @@ def test_single_step_calls_match_one_call(swarm_cls, precision):
     assert np.array_equal(fa, fb) and np.array_equal(pa, pb) and la == lb
 
 
+def test_timing_select(swarm_cls):
+    """With a kernel-name prefix selected, only the matching launches are timed."""
+    _say("timing select")
+    d = W.config(5, scale=0.02)
+    d["pos"] = d["pos"][:20000]
+    with swarm_cls(precision="fp32") as g:
+        g.upload_grid(d["IM"], d["JM"], d["KM"], d["coor"])
+        g.upload_ucat(d["ucat"])
+        g.upload_particles(d["pos"])
+        g.LocateAllParticlesInGrid()
+        g.InterpolateAllFieldsToSwarm()
+        g.timing(True)
+        g.timing_select("step_fused")
+        g.timing_reset()
+        g.AdvanceSimulation(d["dt"], 3, rk2=True, deposit_faces=True)
+        g.synchronize()
+        ms, n = g.timing_query("step_fused_fp32")
+        assert n == 3 and ms > 0
+        assert g.timing_query("deposit_faces")[1] == 0 and g.timing_query("center_to_corner")[1] == 0
+        g.timing_select("")
+        g.timing_reset()
+        g.AdvanceSimulation(d["dt"], 2, rk2=True, deposit_faces=True)
+        g.synchronize()
+        assert g.timing_query("step_fused_fp32")[1] == 2 and g.timing_query("deposit_faces")[1] == 2
+        g.timing(False)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_long_fused_run_matches_staged(swarm_cls, precision):
     """20 steps in one call (the fused path compacts the swarm mid-call) give
