@@ -102,9 +102,11 @@ def measured_traffic(workload: str, variant: str, kernel: str) -> float | None:
     return None
 
 
-def calibration_copy(nbytes: int = 1 << 30) -> None:
+def calibration_copy(nbytes: int = 1 << 30) -> float:
     """A plain device copy of known size, so that counter runs of this command
-    can scale FETCH_SIZE / WRITE_SIZE to bytes."""
+    can scale FETCH_SIZE / WRITE_SIZE to bytes; then five more, timed with
+    events, for the measured copy bandwidth (bytes read + written per second,
+    GB/s), the roofline's second denominator."""
     import torch
 
     src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
@@ -112,9 +114,17 @@ def calibration_copy(nbytes: int = 1 << 30) -> None:
     torch.cuda.synchronize()
     dst.copy_(src)
     torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    bw = 2 * nbytes / (e0.elapsed_time(e1) / 5 * 1e-3) / 1e9
     del src, dst
     torch.cuda.empty_cache()
-    log(f"calibration copy: {nbytes} bytes read, {nbytes} bytes written")
+    log(f"calibration copy: {nbytes} bytes read, {nbytes} bytes written; measured copy bandwidth {bw:.0f} GB/s")
+    return bw
 
 
 def make_inputs(cfg: int, scale: float, device: str):
@@ -472,7 +482,7 @@ def main():
         d["ucont"] = W.contravariant_from_cartesian(d["coor"], d["ucat"], d["IM"], d["JM"], d["KM"]).contiguous()
         d["workload_name"] += " fields"
     torch.cuda.synchronize()
-    calibration_copy()
+    copy_gbs = calibration_copy()
     log(f"inputs: config {args.config} {d['IM']}x{d['JM']}x{d['KM']} cells, {d['n']} particles, "
         f"{time.perf_counter() - t0:.1f}s")
     steps = args.steps if args.steps is not None else min(d["steps"], 100)
@@ -500,6 +510,10 @@ def main():
                               "achieved_GBs": step_bytes(int(n_avg), g, rs, "faces" in d["deposit"]) /
                               (r["ms_per_step"] * 1e-3) / 1e9}
         r["step_roofline"]["frac"] = r["step_roofline"]["achieved_GBs"] / HBM_PEAK_GBS
+        # the measured copy bandwidth as the second denominator
+        r["roofline"]["measured_copy_GBs"] = copy_gbs
+        r["roofline"]["frac_of_measured_copy"] = r["roofline"]["achieved"] / copy_gbs
+        r["step_roofline"]["frac_of_measured_copy"] = r["step_roofline"]["achieved_GBs"] / copy_gbs
         results[v] = r
         torch.cuda.empty_cache()
 
@@ -539,6 +553,7 @@ def main():
                    "eulerian_preparation_each_step": bool(args.fields)},
         "roofline": {k: head["roofline"][k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
         "roofline_kernel": head["roofline"]["kernel"],
+        "measured_copy_GBs": copy_gbs,
         "cpu_baseline": cpu,
         "variants": {k: {kk: vv for kk, vv in v.items() if kk not in ("kernels",)} for k, v in results.items()},
         "kernels": {k: v["kernels"] for k, v in results.items()},
