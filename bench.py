@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/sec (locate+interp+advect+deposit), FP32+MLP vs plain FP64, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+BF16_DENSE_PEAK_TFLOPS = 2500.0
 
 
 def log(*a):
@@ -178,12 +179,20 @@ def run_variant(name: str, d: dict, steps: int, warmup: int, mlp: dict | None):
     res = {"variant": name, "precision": precision, "mlp": bool(use_mlp)}
     if use_mlp:
         res["mlp_train"] = mlp["train"](g)
+        # the guess kernel of the initial locate is timed with events (one launch)
+        g.timing(True)
+        g.timing_select("mlp_guess")
+        g.timing_reset()
     t0 = time.perf_counter()
     g.LocateAllParticlesInGrid()
     st0 = g.stats()
     g.InterpolateAllFieldsToSwarm()
     torch.cuda.synchronize()
     res["initial_locate_s"] = time.perf_counter() - t0
+    if use_mlp:
+        res["mlp_mfma"] = mlp_mfma(res["mlp_train"], g.timing_query("mlp_guess"), n0)
+        g.timing(False)
+        g.timing_select("")
     res["initial_located"], res["initial_lost"] = st0["located"], st0["lost"]
     res["initial_walk_iterations_per_particle"] = st0["walk_iterations"] / max(1, n0)
     log(f"{name}: initial locate {res['initial_locate_s']:.2f}s located={st0['located']} lost={st0['lost']}")
@@ -347,6 +356,27 @@ def precision_errors(d: dict, sample: int) -> dict:
     }
 
 
+def mlp_mfma(train: dict, guess: tuple, n: int) -> dict:
+    """FLOP rates of the MLP kernels against the BF16 dense MFMA peak: the
+    training forward+backward (batch samples per launch) and the start-cell
+    guess of the initial locate (n samples, one launch).  FLOPs are the
+    network's dense products, whether they run on MFMA or the VALU."""
+    from fdf_amd import mlp as M
+
+    out = {"peak": BF16_DENSE_PEAK_TFLOPS, "unit": "TFLOP/s"}
+    rows = []
+    fb = train.get("kernels", {}).get("mlp_fwd_bwd")
+    if fb and fb[1]:
+        rows.append(("train_fwd_bwd", "k_mlp_fwd_bwd", train["batch"] * M.train_flops(), fb[0] / fb[1], fb[1]))
+    if guess[1]:
+        rows.append(("guess", "k_mlp_guess", n * M.forward_flops(), guess[0] / guess[1], guess[1]))
+    for name, kern, flops, avg_ms, cnt in rows:
+        tf = flops / (avg_ms * 1e-3) / 1e12
+        out[name] = {"kernel": kern, "flops_per_launch": flops, "avg_ms": avg_ms, "launches": cnt,
+                     "achieved": tf, "frac": tf / BF16_DENSE_PEAK_TFLOPS}
+    return out
+
+
 def roofline_for(res: dict, d: dict, g: dict) -> dict:
     rsize = 8 if res["precision"] == "fp64" else 4
     kern = res["dominant_kernel_timed"]  # the step kernel, timed over the timed steps
@@ -495,7 +525,7 @@ def main():
     try:
         from fdf_amd import mlp as M
 
-        mlp = {"train": lambda ctx: M.train_on_swarm(ctx, d)}
+        mlp = {"train": lambda ctx: M.train_on_swarm(ctx, d, kernel_timing=True)}
     except Exception as e:  # the MLP module is optional in this bench line only
         log("mlp unavailable:", e)
 
@@ -554,6 +584,7 @@ def main():
         "roofline": {k: head["roofline"][k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
         "roofline_kernel": head["roofline"]["kernel"],
         "measured_copy_GBs": copy_gbs,
+        "mlp_mfma": head.get("mlp_mfma"),
         "cpu_baseline": cpu,
         "variants": {k: {kk: vv for kk, vv in v.items() if kk not in ("kernels",)} for k, v in results.items()},
         "kernels": {k: v["kernels"] for k, v in results.items()},

@@ -146,61 +146,127 @@ struct MlpNorm {
 };
 __device__ __forceinline__ float norm_axis(double x, double lo, double s) { return (float)((x - lo) * s - 1.0); }
 
-// Start cells for particles without a valid cell (cell < 0); others get -1.
-__global__ __launch_bounds__(MLP_THREADS) void k_mlp_guess(int64_t n, const double *__restrict__ pos,
-                                                           const int32_t *__restrict__ cell, MlpWeights Wt,
-                                                           MlpNorm nm, int IM, int JM, int KM,
-                                                           int32_t *__restrict__ guess) {
-  __shared__ TileLds L;
-  const int64_t base = (int64_t)blockIdx.x * TILE;
-  const int l = threadIdx.x;
-  // a tile is skipped when none of its particles needs a guess (uniform per block)
-  int need = 0;
-  if (l < TILE && base + l < n) need = (cell == nullptr || cell[base + l] < 0) ? 1 : 0;
-  if (!__syncthreads_or(need)) {
-    if (l < TILE && base + l < n) guess[base + l] = -1;
-    return;
-  }
-  if (l < TILE) {
-    const int64_t p = base + l;
-    if (p < n) {
-      L.x[l][0] = norm_axis(pos[3 * p], nm.lo0, nm.s0);
-      L.x[l][1] = norm_axis(pos[3 * p + 1], nm.lo1, nm.s1);
-      L.x[l][2] = norm_axis(pos[3 * p + 2], nm.lo2, nm.s2);
-    } else {
-      L.x[l][0] = L.x[l][1] = L.x[l][2] = 0.f;
-    }
+// Inference of 32 samples per wavefront with the weights held in registers.
+// Layers 2 and 3 run transposed (Z2^T = W2 A1^T, Y^T = W3 A2^T), so each
+// layer's MFMA result is already in the operand layout of the next one:
+// lane l works on sample l%32 and, with h = l/32, holds hidden units
+// 8kk + 4h + v (kk = 0..7, v = 0..3) as the B operand of a product; the
+// result register 4q + v of column tile t is hidden unit 32t + 8q + 4h + v,
+// which is B-operand slot (kk = 4t + q, v) of the next product.  Rounding and
+// summation order are those of tile_forward: layer 1 as the same FP32
+// expression, products over k in ascending blocks of 8, biases added after.
+constexpr int INFER_WAVES = 4;      // wavefronts per block
+constexpr int INFER_BLOCKS = 1024;  // the waves loop over 32-sample tiles
+
+// GUESS: start cells from positions (cell < 0 or no cell array; others get
+// -1, and tiles where nobody needs one skip the network).  Otherwise the
+// network's outputs for normalised inputs xin (tests).
+template <bool GUESS>
+__global__ __launch_bounds__(64 * INFER_WAVES) void k_mlp_infer(int64_t n, const double *__restrict__ pos,
+                                                                const int32_t *__restrict__ cell,
+                                                                const float *__restrict__ xin, MlpWeights Wt,
+                                                                MlpNorm nm, int IM, int JM, int KM,
+                                                                int32_t *__restrict__ guess, float *__restrict__ yout) {
+  __shared__ float4 w1b[H];  // layer 1: (w0, w1, w2, bias) of each hidden unit
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r32 = l & 31, h = l >> 5;
+  if (threadIdx.x < H) {
+    const int q = threadIdx.x;
+    w1b[q] = make_float4(bf2f(Wt.wb[OFF_W1 + 3 * q]), bf2f(Wt.wb[OFF_W1 + 3 * q + 1]),
+                         bf2f(Wt.wb[OFF_W1 + 3 * q + 2]), Wt.p[OFF_B1 + q]);
   }
   __syncthreads();
-  tile_forward(Wt, L);
-  if (l < TILE && base + l < n) {
-    const int64_t p = base + l;
-    int32_t g = -1;
-    if (cell == nullptr || cell[p] < 0) {
-      int i = (int)floorf(L.y[l][0] * (float)IM), j = (int)floorf(L.y[l][1] * (float)JM),
-          k = (int)floorf(L.y[l][2] * (float)KM);
-      i = i < 0 ? 0 : (i >= IM ? IM - 1 : i);
-      j = j < 0 ? 0 : (j >= JM ? JM - 1 : j);
-      k = k < 0 ? 0 : (k >= KM ? KM - 1 : k);
-      g = (int32_t)(i + IM * (j + JM * k));
+  bf16x4 w2f[2][8], w3f[8];
+  float b2r[2][16];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) w2f[t][kk][v] = (short)Wt.wb[OFF_W2 + (32 * t + r32) * H + 8 * kk + 4 * h + v];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) b2r[t][r] = Wt.p[OFF_B2 + 32 * t + 8 * (r / 4) + 4 * h + (r % 4)];
+  }
+#pragma unroll
+  for (int kk = 0; kk < 8; ++kk)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) w3f[kk][v] = r32 < 3 ? (short)Wt.wb[OFF_W3 + r32 * H + 8 * kk + 4 * h + v] : (short)0;
+  const float b30 = Wt.p[OFF_B3], b31 = Wt.p[OFF_B3 + 1], b32 = Wt.p[OFF_B3 + 2];
+
+  const int64_t tiles = (n + 31) / 32;
+  // tile is the same in every lane of a wave, so the loop and the branches on
+  // __ballot below are wave-uniform (MFMA runs with all lanes)
+  for (int64_t tile = (int64_t)blockIdx.x * INFER_WAVES + wv; tile < tiles;
+       tile += (int64_t)gridDim.x * INFER_WAVES) {
+    const int64_t p = tile * 32 + r32;
+    const bool in = p < n;
+    bool need = in;
+    if (GUESS) need = in && (cell == nullptr || cell[p] < 0);
+    if (__ballot(need) == 0) {
+      if (GUESS && h == 0 && in) guess[p] = -1;
+      continue;
     }
-    guess[p] = g;
+    float x0 = 0.f, x1 = 0.f, x2 = 0.f;
+    if (in) {
+      if (GUESS) {
+        x0 = norm_axis(pos[3 * p], nm.lo0, nm.s0);
+        x1 = norm_axis(pos[3 * p + 1], nm.lo1, nm.s1);
+        x2 = norm_axis(pos[3 * p + 2], nm.lo2, nm.s2);
+      } else {
+        x0 = xin[3 * p];
+        x1 = xin[3 * p + 1];
+        x2 = xin[3 * p + 2];
+      }
+    }
+    // layer 1 (VALU), straight into the B operand of layer 2
+    bf16x4 a1f[8];
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float4 w = w1b[8 * kk + 4 * h + v];
+        const float z = __builtin_fmaf(w.x, x0, __builtin_fmaf(w.y, x1, w.z * x2)) + w.w;
+        a1f[kk][v] = (short)f2bf(gelu(z));
+      }
+    // layer 2 (MFMA): column tile t holds hidden units 32t..32t+31
+    bf16x4 a2f[8];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      f32x16 acc = {};
+#pragma unroll
+      for (int kk = 0; kk < 8; ++kk) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w2f[t][kk], a1f[kk], acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) a2f[4 * t + r / 4][r % 4] = (short)f2bf(gelu(acc[r] + b2r[t][r]));
+    }
+    // layer 3 (MFMA): rows 0..2 of the result (registers 0..2 of lanes 0..31)
+    f32x16 acc3 = {};
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) acc3 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w3f[kk], a2f[kk], acc3, 0, 0, 0);
+    if (h == 0 && in) {
+      const float y0 = acc3[0] + b30, y1 = acc3[1] + b31, y2 = acc3[2] + b32;
+      if (GUESS) {
+        int32_t g = -1;
+        if (need) {
+          int i = (int)floorf(y0 * (float)IM), j = (int)floorf(y1 * (float)JM), k = (int)floorf(y2 * (float)KM);
+          i = i < 0 ? 0 : (i >= IM ? IM - 1 : i);
+          j = j < 0 ? 0 : (j >= JM ? JM - 1 : j);
+          k = k < 0 ? 0 : (k >= KM ? KM - 1 : k);
+          g = (int32_t)(i + IM * (j + JM * k));
+        }
+        guess[p] = g;
+      } else {
+        yout[3 * p] = y0;
+        yout[3 * p + 1] = y1;
+        yout[3 * p + 2] = y2;
+      }
+    }
   }
 }
 
-// forward on normalised inputs (tests)
-__global__ __launch_bounds__(MLP_THREADS) void k_mlp_forward(int64_t n, const float *__restrict__ x, MlpWeights Wt,
-                                                             float *__restrict__ y) {
-  __shared__ TileLds L;
-  const int64_t base = (int64_t)blockIdx.x * TILE;
-  const int l = threadIdx.x;
-  if (l < TILE) {
-    for (int c = 0; c < 3; ++c) L.x[l][c] = base + l < n ? x[3 * (base + l) + c] : 0.f;
-  }
-  __syncthreads();
-  tile_forward(Wt, L);
-  if (l < TILE && base + l < n)
-    for (int c = 0; c < 3; ++c) y[3 * (base + l) + c] = L.y[l][c];
+static int64_t infer_blocks(int64_t n) {
+  const int64_t tiles = (n + 31) / 32;
+  const int64_t b = (tiles + INFER_WAVES - 1) / INFER_WAVES;
+  return b < INFER_BLOCKS ? b : INFER_BLOCKS;
 }
 
 // ---------------------------------------------------------------- training
@@ -416,9 +482,8 @@ int mlp_guess_device(picgpu_ctx *c, int64_t n, const double *d_pos, const int32_
     return PICGPU_E_STATE;
   }
   if (n <= 0) return PICGPU_OK;
-  const int64_t tiles = (n + TILE - 1) / TILE;
-  PG_LAUNCH(c, "mlp_guess", tiles, MLP_THREADS, k_mlp_guess, n, d_pos, d_cell, weights(c), norm_of(c), c->IM, c->JM,
-            c->KM, d_guess);
+  PG_LAUNCH(c, "mlp_guess", infer_blocks(n), 64 * INFER_WAVES, k_mlp_infer<true>, n, d_pos, d_cell,
+            (const float *)nullptr, weights(c), norm_of(c), c->IM, c->JM, c->KM, d_guess, (float *)nullptr);
   return PICGPU_OK;
 }
 
@@ -499,7 +564,8 @@ int picgpu_mlp_forward(picgpu_ctx *c, int64_t n, const float *x, float *y) {
   PG_CHECK(c->mlp_x.ensure(sizeof(float) * 3 * (size_t)n));
   PG_CHECK(c->mlp_y.ensure(sizeof(float) * 3 * (size_t)n));
   PG_HIP(hipMemcpy(c->mlp_x.ptr, x, sizeof(float) * 3 * (size_t)n, hipMemcpyHostToDevice));
-  PG_LAUNCH(c, "mlp_forward", (n + TILE - 1) / TILE, MLP_THREADS, k_mlp_forward, n, c->mlp_x.as<float>(), weights(c),
+  PG_LAUNCH(c, "mlp_forward", infer_blocks(n), 64 * INFER_WAVES, k_mlp_infer<false>, n, (const double *)nullptr,
+            (const int32_t *)nullptr, c->mlp_x.as<float>(), weights(c), MlpNorm{}, 1, 1, 1, (int32_t *)nullptr,
             c->mlp_y.as<float>());
   PG_HIP(hipStreamSynchronize(c->stream));
   PG_HIP(hipMemcpy(y, c->mlp_y.ptr, sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToHost));

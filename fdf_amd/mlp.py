@@ -17,18 +17,24 @@ from .swarm import PicGpu
 
 
 def train_from_walk(ctx: PicGpu, d: dict, n_pairs: int | None = None, iters: int = 1500, batch: int = 4096,
-                    lr: float = 3e-3, seed: int = 3004, rounds: int = 2) -> dict:
+                    lr: float = 3e-3, seed: int = 3004, rounds: int = 2, kernel_timing: bool = False) -> dict:
     """Train ctx's MLP on pairs located by the walk for the first n_pairs
     particles of workload d (device positions d["pos_dev"] or host d["pos"]).
 
     Round 1 locates with the reference start rule; on curvilinear grids that
     start (cell (0,0,0)) loses many walks, so each later round re-locates the
     still-LOST particles from the current network's guess and trains again
-    on the enlarged set of walk-produced pairs."""
+    on the enlarged set of walk-produced pairs.  With kernel_timing the
+    training kernels are timed with HIP events and the result holds their
+    totals under "kernels" ({name: (total ms, launches)})."""
     n_pairs = int(n_pairs or d.get("train_pairs") or min(2_097_152, d["n"]))
     tmp = PicGpu(precision=ctx.precision)
     hist = []
+    kernels = {}
     try:
+        if kernel_timing:
+            tmp.timing(True)
+            tmp.timing_select("mlp_")
         if "pos_dev" in d:
             tmp.upload_grid_device(d["IM"], d["JM"], d["KM"], d["coor"].data_ptr())
             n_pairs = min(n_pairs, int(d["pos_dev"].shape[0]))
@@ -56,12 +62,31 @@ def train_from_walk(ctx: PicGpu, d: dict, n_pairs: int | None = None, iters: int
             if st["lost"] == 0:
                 break
         params = tmp.mlp_get_params()
+        if kernel_timing:
+            tmp.synchronize()
+            for k in ("mlp_fwd_bwd", "mlp_reduce", "mlp_adam", "mlp_guess"):
+                kernels[k] = tmp.timing_query(k)
     finally:
         tmp.close()
     ctx.mlp_set_params(params)
     ctx.mlp_enable(True)
-    return {"pairs": int(located), "walk_s": walk_s, "train_s": train_s, "iters_per_round": iters,
-            "batch": batch, "final_mse": float(loss), "rounds": hist}
+    out = {"pairs": int(located), "walk_s": walk_s, "train_s": train_s, "iters_per_round": iters,
+           "batch": batch, "final_mse": float(loss), "rounds": hist}
+    if kernel_timing:
+        out["kernels"] = kernels
+    return out
+
+
+def forward_flops(H: int = 64) -> int:
+    """Dense-product FLOPs of one sample's forward pass (3->H->H->3)."""
+    return 2 * (3 * H + H * H + H * 3)
+
+
+def train_flops(H: int = 64) -> int:
+    """Dense-product FLOPs of one sample's forward and backward pass: the
+    forward, then dW3, dA2, dW2, dA1 and dW1 (the input gradient is not
+    formed)."""
+    return forward_flops(H) + 2 * (H * 3 + H * 3 + H * H + H * H + 3 * H)
 
 
 def train_on_swarm(ctx: PicGpu, d: dict, **kw) -> dict:
