@@ -1,6 +1,7 @@
 // mlp.hip -- the learned initial guess for the walking search (gfx950).
 //
-// A 3 -> 64 -> 64 -> 3 MLP with exact (erf) GELU maps normalised physical
+// A 3 -> 64 -> 64 -> 3 MLP with erf GELU (erff in training; in inference a
+// branch-free erf within 1.5e-7, gelu_as) maps normalised physical
 // coordinates to the fractional logical position ((i+a1)/IM, (j+a2)/JM,
 // (k+a3)/KM).  Weights are BF16, products accumulate in FP32; layer 1 runs on
 // the VALU (K = 3), layers 2 and 3 and the large backward products on MFMA
@@ -146,7 +147,30 @@ struct MlpNorm {
 };
 __device__ __forceinline__ float norm_axis(double x, double lo, double s) { return (float)((x - lo) * s - 1.0); }
 
-// Inference of 32 samples per wavefront with the weights held in registers.
+// FP32 -> BF16 bits, rounded to nearest even like f2bf; on gfx950 a single
+// v_cvt_pk_bf16_f32 instead of f2bf's integer sequence.
+__device__ __forceinline__ short bf16_rne(float f) { return __builtin_bit_cast(short, (__bf16)f); }
+
+// GELU with erf(u) = 1 - P(t) exp(-u^2), t = 1 / (1 + p u), u >= 0
+// (Abramowitz & Stegun 7.1.26, |error| <= 1.5e-7), written so that neither
+// tail cancels: with q = P(t) exp(-u^2) at |u|, 1 + erf(u) is 2 - q for
+// u >= 0 and q for u < 0.  The reciprocal and exp2 are the hardware
+// approximations (v_rcp_f32, v_exp_f32, about 1 ulp each).
+__device__ __forceinline__ float gelu_as(float z) {
+  const float u = z * 0.70710678118654752f;
+  const float au = fabsf(u);
+  const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(0.3275911f, au, 1.0f));
+  float P = __builtin_fmaf(1.061405429f, t, -1.453152027f);
+  P = __builtin_fmaf(P, t, 1.421413741f);
+  P = __builtin_fmaf(P, t, -0.284496736f);
+  P = __builtin_fmaf(P, t, 0.254829592f);
+  P = P * t;
+  const float q = P * __builtin_amdgcn_exp2f(-(au * au) * 1.44269504088896341f);
+  return 0.5f * z * (u >= 0.f ? 2.0f - q : q);
+}
+
+// Inference of 32 samples per wavefront; the weights are staged once per
+// block in LDS, in the operand layout of the products.
 // Layers 2 and 3 run transposed (Z2^T = W2 A1^T, Y^T = W3 A2^T), so each
 // layer's MFMA result is already in the operand layout of the next one:
 // lane l works on sample l%32 and, with h = l/32, holds hidden units
@@ -155,42 +179,51 @@ __device__ __forceinline__ float norm_axis(double x, double lo, double s) { retu
 // which is B-operand slot (kk = 4t + q, v) of the next product.  Rounding and
 // summation order are those of tile_forward: layer 1 as the same FP32
 // expression, products over k in ascending blocks of 8, biases added after.
+//
+// Inference evaluates GELU with gelu_as instead of erff: the same
+// 0.5 z (1 + erf(z / sqrt 2)), with erf from Abramowitz & Stegun 7.1.26.
+// Its erf error is at most 1.5e-7 absolute, below the BF16 rounding that
+// follows, and it has no branches (erff takes one of two paths per lane, and
+// a wavefront runs both).
 constexpr int INFER_WAVES = 4;      // wavefronts per block
-constexpr int INFER_BLOCKS = 1024;  // the waves loop over 32-sample tiles
+constexpr int INFER_BLOCKS = 1536;  // 6 wavefronts per SIMD; they loop over 32-sample tiles
 
 // GUESS: start cells from positions (cell < 0 or no cell array; others get
 // -1, and tiles where nobody needs one skip the network).  Otherwise the
 // network's outputs for normalised inputs xin (tests).
 template <bool GUESS>
-__global__ __launch_bounds__(64 * INFER_WAVES) void k_mlp_infer(int64_t n, const double *__restrict__ pos,
+__global__ __launch_bounds__(64 * INFER_WAVES, 4) void k_mlp_infer(int64_t n, const double *__restrict__ pos,
                                                                 const int32_t *__restrict__ cell,
                                                                 const float *__restrict__ xin, MlpWeights Wt,
                                                                 MlpNorm nm, int IM, int JM, int KM,
                                                                 int32_t *__restrict__ guess, float *__restrict__ yout) {
-  __shared__ float4 w1b[H];  // layer 1: (w0, w1, w2, bias) of each hidden unit
+  __shared__ float4 w1b[H];          // layer 1: (w0, w1, w2, bias) of each hidden unit
+  __shared__ bf16x4 w2s[2][8][64];   // layer 2 A-operand fragments, by (column tile, k block, lane)
+  __shared__ float b2s[H];           // layer 2 biases
+  __shared__ bf16x4 w3s[8][64];      // layer 3 A-operand fragments (rows 3..31 zero), by (k block, lane)
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r32 = l & 31, h = l >> 5;
   if (threadIdx.x < H) {
     const int q = threadIdx.x;
     w1b[q] = make_float4(bf2f(Wt.wb[OFF_W1 + 3 * q]), bf2f(Wt.wb[OFF_W1 + 3 * q + 1]),
                          bf2f(Wt.wb[OFF_W1 + 3 * q + 2]), Wt.p[OFF_B1 + q]);
+    b2s[q] = Wt.p[OFF_B2 + q];
+  }
+  for (int e = threadIdx.x; e < 2 * 8 * 64; e += blockDim.x) {
+    const int t = e / 512, kk = (e / 64) % 8, ln = e % 64;
+    bf16x4 f;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) f[v] = (short)Wt.wb[OFF_W2 + (32 * t + (ln & 31)) * H + 8 * kk + 4 * (ln >> 5) + v];
+    w2s[t][kk][ln] = f;
+  }
+  for (int e = threadIdx.x; e < 8 * 64; e += blockDim.x) {
+    const int kk = e / 64, ln = e % 64, c = ln & 31;
+    bf16x4 f;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) f[v] = c < 3 ? (short)Wt.wb[OFF_W3 + c * H + 8 * kk + 4 * (ln >> 5) + v] : (short)0;
+    w3s[kk][ln] = f;
   }
   __syncthreads();
-  bf16x4 w2f[2][8], w3f[8];
-  float b2r[2][16];
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-#pragma unroll
-    for (int kk = 0; kk < 8; ++kk)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) w2f[t][kk][v] = (short)Wt.wb[OFF_W2 + (32 * t + r32) * H + 8 * kk + 4 * h + v];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) b2r[t][r] = Wt.p[OFF_B2 + 32 * t + 8 * (r / 4) + 4 * h + (r % 4)];
-  }
-#pragma unroll
-  for (int kk = 0; kk < 8; ++kk)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) w3f[kk][v] = r32 < 3 ? (short)Wt.wb[OFF_W3 + r32 * H + 8 * kk + 4 * h + v] : (short)0;
   const float b30 = Wt.p[OFF_B3], b31 = Wt.p[OFF_B3 + 1], b32 = Wt.p[OFF_B3 + 2];
 
   const int64_t tiles = (n + 31) / 32;
@@ -226,7 +259,7 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_mlp_infer(int64_t n, const
       for (int v = 0; v < 4; ++v) {
         const float4 w = w1b[8 * kk + 4 * h + v];
         const float z = __builtin_fmaf(w.x, x0, __builtin_fmaf(w.y, x1, w.z * x2)) + w.w;
-        a1f[kk][v] = (short)f2bf(gelu(z));
+        a1f[kk][v] = bf16_rne(gelu_as(z));
       }
     // layer 2 (MFMA): column tile t holds hidden units 32t..32t+31
     bf16x4 a2f[8];
@@ -234,14 +267,14 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_mlp_infer(int64_t n, const
     for (int t = 0; t < 2; ++t) {
       f32x16 acc = {};
 #pragma unroll
-      for (int kk = 0; kk < 8; ++kk) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w2f[t][kk], a1f[kk], acc, 0, 0, 0);
+      for (int kk = 0; kk < 8; ++kk) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w2s[t][kk][l], a1f[kk], acc, 0, 0, 0);
 #pragma unroll
-      for (int r = 0; r < 16; ++r) a2f[4 * t + r / 4][r % 4] = (short)f2bf(gelu(acc[r] + b2r[t][r]));
+      for (int r = 0; r < 16; ++r) a2f[4 * t + r / 4][r % 4] = bf16_rne(gelu_as(acc[r] + b2s[32 * t + 8 * (r / 4) + 4 * h + (r % 4)]));
     }
     // layer 3 (MFMA): rows 0..2 of the result (registers 0..2 of lanes 0..31)
     f32x16 acc3 = {};
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) acc3 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w3f[kk], a2f[kk], acc3, 0, 0, 0);
+    for (int kk = 0; kk < 8; ++kk) acc3 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w3s[kk][l], a2f[kk], acc3, 0, 0, 0);
     if (h == 0 && in) {
       const float y0 = acc3[0] + b30, y1 = acc3[1] + b31, y2 = acc3[2] + b32;
       if (GUESS) {
