@@ -2,8 +2,9 @@
 
 The oracle applies the device's rounding rule (BF16 weights and BF16 matmul
 inputs a1, a2, g2; FP64 elsewhere), so the remaining differences come from
-FP32 accumulation and FP32 erf/exp: a rare one-ulp flip of a BF16-rounded
-activation moves one output by ~1e-3.  Tolerances are written per test.
+FP32 accumulation and FP32 erf/exp (in inference, an erf within 1.5e-7 of
+the exact one): a rare one-ulp flip of a BF16-rounded activation moves one
+output by ~1e-3.  Tolerances are written per test.
 """
 import numpy as np
 import pytest
@@ -49,6 +50,26 @@ def test_mlp_forward_matches_oracle(ctx, oracle):
     # the same samples in another order give the same outputs (no cross-sample coupling)
     perm = np.random.default_rng(3).permutation(x.shape[0])
     assert np.array_equal(ctx.mlp_forward(x[perm]), y[perm])
+
+
+def test_mlp_forward_many_tiles_per_wavefront(ctx, oracle):
+    """Above 1536 blocks x 4 wavefronts x 32 = 196,608 samples each wavefront
+    of the inference kernel loops over several 32-sample tiles; this size
+    gives three or four tiles per wavefront and a ragged last tile."""
+    _say("mlp_forward many tiles")
+    p = M.initial_params(8)
+    ctx.mlp_set_params(p)
+    x = np.random.default_rng(9).uniform(-1, 1, (3 * 196_608 + 29, 3)).astype(np.float32)
+    y = ctx.mlp_forward(x)
+    yo = oracle.mlp_forward(p.astype(np.float64), x.astype(np.float64), round_bf16=True)
+    err = np.abs(y - yo)
+    _say("forward err median", np.median(err), "max", err.max())
+    assert np.median(err) < 1e-5
+    assert err.max() < 5e-3
+    # samples from late in the waves' loops, shifted to other lanes and tiles
+    # in a small launch, give the same outputs
+    tail = 4096 + 7
+    assert np.array_equal(ctx.mlp_forward(x[-tail:]), y[-tail:])
 
 
 def test_mlp_gradient_matches_oracle(ctx, oracle):
