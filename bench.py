@@ -369,7 +369,7 @@ def mlp_mfma(train: dict, guess: tuple, n: int) -> dict:
     if fb and fb[1]:
         rows.append(("train_fwd_bwd", "k_mlp_fwd_bwd", train["batch"] * M.train_flops(), fb[0] / fb[1], fb[1]))
     if guess[1]:
-        rows.append(("guess", "k_mlp_guess", n * M.forward_flops(), guess[0] / guess[1], guess[1]))
+        rows.append(("guess", "k_mlp_infer<true>", n * M.forward_flops(), guess[0] / guess[1], guess[1]))
     for name, kern, flops, avg_ms, cnt in rows:
         tf = flops / (avg_ms * 1e-3) / 1e12
         out[name] = {"kernel": kern, "flops_per_launch": flops, "avg_ms": avg_ms, "launches": cnt,
