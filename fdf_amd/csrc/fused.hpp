@@ -748,19 +748,25 @@ __device__ __forceinline__ int32_t bucket_of(int32_t c, int32_t st, int IM, int 
   return st == ST_REMOVED ? -1 : nkeys;
 }
 
+// guess (may be null): the start cell of a particle without a cell, so that
+// the index orders a fresh locate by where its walks start.
 __global__ __launch_bounds__(kBlock) void k_key_histogram(int64_t n, const int32_t *__restrict__ cell,
-                                                          const int32_t *__restrict__ status, int IM, int JM,
+                                                          const int32_t *__restrict__ status,
+                                                          const int32_t *__restrict__ guess, int IM, int JM,
                                                           TileMap tm, int32_t nkeys, int32_t *__restrict__ count,
                                                           int32_t *__restrict__ slot) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const int32_t b = bucket_of(cell[p], status[p], IM, JM, tm, nkeys);
+  int32_t c = cell[p];
+  if (c < 0 && guess) c = guess[p];
+  const int32_t b = bucket_of(c, status[p], IM, JM, tm, nkeys);
   slot[p] = b >= 0 ? atomicAdd(&count[b], 1) : -1;
 }
 
 // idx[kstart[bucket] + slot] = p
 __global__ __launch_bounds__(kBlock) void k_key_scatter(int64_t n, const int32_t *__restrict__ cell,
                                                         const int32_t *__restrict__ status,
+                                                        const int32_t *__restrict__ guess,
                                                         const int32_t *__restrict__ slot, int IM, int JM, TileMap tm,
                                                         int32_t nkeys, const int32_t *__restrict__ kstart,
                                                         int32_t *__restrict__ idx) {
@@ -768,7 +774,9 @@ __global__ __launch_bounds__(kBlock) void k_key_scatter(int64_t n, const int32_t
   if (p >= n) return;
   const int32_t s = slot[p];
   if (s < 0) return;
-  const int32_t b = bucket_of(cell[p], status[p], IM, JM, tm, nkeys);
+  int32_t c = cell[p];
+  if (c < 0 && guess) c = guess[p];
+  const int32_t b = bucket_of(c, status[p], IM, JM, tm, nkeys);
   idx[kstart[b] + s] = (int32_t)p;
 }
 

@@ -93,6 +93,7 @@ struct LocateArgs {
   R *w;
   int32_t *status;
   const int32_t *guess;  // may be null
+  const int32_t *order;  // may be null: the q-th particle visited is order[q]
   int32_t *steps_out;    // may be null
   unsigned long long *walk_hist;  // null, or [kCounterStripes][64] walk-iteration histogram
   unsigned long long *counters;
@@ -123,15 +124,16 @@ __global__ __launch_bounds__(kBlock) void k_locate(LocateArgs<R> a) {
       if (p < 0) {
         int64_t q = cursor + __popcll(idle & lane_mask_lt);
         if (q < end) {
-          int32_t st = a.status[q];
+          const int64_t pq = a.order ? (int64_t)a.order[q] : q;
+          int32_t st = a.status[pq];
           bool skip = (st == ST_LOCATED) || (a.skip_lost && st == ST_LOST);
           if (!skip) {
-            p = q;
-            x[0] = a.pos[3 * q];
-            x[1] = a.pos[3 * q + 1];
-            x[2] = a.pos[3 * q + 2];
-            int32_t c = a.cell[q];
-            if (c < 0 && a.guess) c = a.guess[q];
+            p = pq;
+            x[0] = a.pos[3 * pq];
+            x[1] = a.pos[3 * pq + 1];
+            x[2] = a.pos[3 * pq + 2];
+            int32_t c = a.cell[pq];
+            if (c < 0 && a.guess) c = a.guess[pq];
             if (c >= 0) {
               unpack_cell(c, cell_idx(a.g), ci, cj, ck);
             } else {
@@ -142,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_locate(LocateArgs<R> a) {
             steps = 0;
             cyc = CycleGuard();
           } else if (a.steps_out) {
-            a.steps_out[q] = 0;
+            a.steps_out[pq] = 0;
           }
         }
       }

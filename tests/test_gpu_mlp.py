@@ -153,3 +153,34 @@ def test_training_saves_walk_steps_and_keeps_ownership(oracle, precision):
         for c in (ca[q], cb[q]):
             V = np.array([c3[c[2] + dk, c[1] + dj, c[0] + di] for di, dj, dk in off])
             assert oracle.evaluate(V, p)[2] >= 0  # both cells accept the point: an overlap
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_guess_locate_grouped_by_start_cell_matches_plain_locate(precision):
+    """An FP64 locate of at least 2**20 particles from guessed start cells
+    visits them grouped by start cell (FP32 in index order).  Each walk is
+    unchanged by the order: cells, weights and statuses equal those of the
+    walks from (0,0,0), and the cells are the closed form floor(32 x) of the
+    uniform grid."""
+    _say("grouped locate", precision)
+    from fdf_amd.swarm import PicGpu
+
+    n = (1 << 20) + 4099
+    pos = np.random.default_rng(21).random((n, 3))
+    res = {}
+    for use in (False, True):
+        with PicGpu(precision=precision) as g:
+            g.upload_grid(32, 32, 32, W.uniform_grid(32, 32, 32))
+            if use:
+                g.mlp_set_params(M.initial_params(13))  # an untrained guess: any start cell
+                g.mlp_enable(True)
+            g.upload_particles(pos)
+            g.LocateAllParticlesInGrid()
+            res[use] = g.download_particles()
+    a, b = res[False], res[True]
+    oa, ob = np.argsort(a["pid"]), np.argsort(b["pid"])
+    for k in ("cell", "weight", "status"):
+        assert np.array_equal(a[k][oa], b[k][ob]), k
+    assert (b["status"] == 1).all()
+    assert np.array_equal(b["pos"][ob], pos)
+    assert (b["cell"][ob] == np.floor(pos * 32).astype(np.int64)).all(axis=1).mean() > 1 - 1e-5
