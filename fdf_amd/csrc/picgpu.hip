@@ -581,7 +581,9 @@ static int do_locate(picgpu_ctx *c, int skip_lost) {
     }
     const int32_t *order = nullptr;
     if (guess && n >= kOrderedLocateMin && c->opt.precision == PICGPU_FP64) {
-      // the key index buffers hold the order; the index is rebuilt before the next fused step
+      // the key index buffers hold the order; the index is rebuilt before the
+      // next fused step (marked stale first, in case the build fails part-way)
+      c->keyed = c->key_ordered = false;
       PG_CHECK(build_key_index(c, guess));
       if (c->n_live == n) order = c->kidx[c->kcur].as<int32_t>();
     }
