@@ -114,6 +114,7 @@ _SIG = {
     "picgpu_interpolate": (C.c_int, [_P]),
     "picgpu_scatter": (C.c_int, [_P]),
     "picgpu_deposit_faces": (C.c_int, [_P, C.c_double]),
+    "picgpu_set_particle_volume": (C.c_int, [_P, C.c_double]),
     "picgpu_step": (C.c_int, [_P, C.c_double, C.c_int, C.c_uint]),
     "picgpu_grid_field_size": (C.c_int, [_P, C.c_int, C.POINTER(_I64)]),
     "picgpu_download_grid_field": (C.c_int, [_P, C.c_int, _P]),

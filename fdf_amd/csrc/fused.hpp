@@ -867,19 +867,18 @@ __global__ __launch_bounds__(kBlock) void k_bucket_pid_order(int64_t nkeys, cons
 // Face deposition (no reference counterpart).  A particle in cell (i,j,k)
 // with weights (a1,a2,a3) adds (1-a1) to i-face i and a1 to i-face i+1, and
 // likewise along j and k; faces sum count and momentum.  The lower face of
-// cell c in direction d sums the lower neighbour's particles (weight a_d),
-// then c's own (weight 1-a_d), each in PID order (buckets ordered by
-// k_bucket_pid_order), one accumulator running through both.
+// cell c in direction d is the sum over the lower neighbour's particles
+// (weight a_d) plus the sum over c's own (weight 1-a_d), each summed from
+// zero in PID order (buckets ordered by k_bucket_pid_order).
 //
 // One workgroup per 8x8x4 tile of the extended range [0..IM]x[0..JM]x[0..KM].
 // Pass 1: each thread runs its cell's particles once and leaves, per
-// direction, the accumulator state after the weight-a sums in LDS; 128
-// threads do the same, for one direction each, for the cells just below the
-// tile's lower faces.  Pass 2: each thread starts its three lower faces from
-// the lower neighbour's state and continues with its own particles (weight
-// 1-a).  A cell's first particles are read once, with independent loads, and
-// kept in registers for both passes; the sums are those of the
-// one-accumulator order above, plain sums (FaceAcc).  Neighbouring threads
+// direction, its weight-a sums in LDS; 128 threads do the same, for one
+// direction each, for the cells just below the tile's lower faces.  Pass 2:
+// each thread sums its own particles with weight 1-a and adds the lower
+// neighbour's sums.  A cell's first particles are read once, with
+// independent loads, and kept in registers for both passes; the sums are
+// plain sums (FaceAcc).  Neighbouring threads
 // take neighbouring cells, so that each load instruction of a wavefront reads
 // a few contiguous lines.  In FP32 the kernel is held to at least 6
 // wavefronts per SIMD (at most 80 VGPRs; it takes 69, so 7 fit, with no
@@ -1017,16 +1016,17 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 6 : 1) void k_deposit_tile
   }
   __syncthreads();
   if (i > IM || j > JM || k > KM) return;
-  // pass 2: lower faces, from the lower neighbour's state through the own particles
+  // pass 2: lower faces, the own particles' sums plus the lower neighbour's
   const bool has[3] = {j < JM && k < KM, i < IM && k < KM, i < IM && j < JM};
   FaceAcc<R> acc[3][4];
+  R lower[3][4];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const bool inside = d == 0 ? li > 0 : d == 1 ? lj > 0 : lk > 0;
     const int nl = l - (d == 0 ? 1 : d == 1 ? TX : TX * TY);  // lower neighbour in the tile
     const int h = d == 0 ? lj + TY * lk : d == 1 ? TY * TZ + li + TX * lk : TY * TZ + TX * TZ + li + TX * lj;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[d][v].s = inside ? us[d][v][nl] : hs[v][h];
+    for (int v = 0; v < 4; ++v) lower[d][v] = inside ? us[d][v][nl] : hs[v][h];
   }
   auto add_1ma = [&](const R wa[3], const R uu[3]) {
 #pragma unroll
@@ -1054,7 +1054,7 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 6 : 1) void k_deposit_tile
   for (int d = 0; d < 3; ++d) {
     if (!has[d]) continue;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) out[d][v] = acc[d][v].value();
+    for (int v = 0; v < 4; ++v) out[d][v] = lower[d][v] + acc[d][v].value();
   }
 }
 

@@ -540,7 +540,7 @@ struct Acc {
 };
 
 // Accumulator of the face deposition: a plain sum in R, in the fixed order
-// (lower cell then upper cell, each in PID order).  A face sums a few dozen
+// (one cell's particles in PID order).  A face sums a few dozen
 // terms at most, so in FP32 its error stays within n * 2^-24 of the summed
 // magnitudes, far inside the FP32 mode's 1e-5; a compensated sum costs seven
 // instructions an addition instead of one.
@@ -573,9 +573,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter_P(int64_t ncell, const int32
 
 // Face deposition (no reference counterpart).  A particle in cell (i,j,k)
 // with weights (a1,a2,a3) adds (1-a1) to i-face i and a1 to i-face i+1, and
-// likewise along j and k; the face sums count and momentum.  Each face
-// visits its lower cell (weight a) then its upper cell (weight 1-a), each in
-// PID order.  dir: 0 = i-faces [KM][JM][IM+1], 1 = j, 2 = k.
+// likewise along j and k; the face sums count and momentum.  A face is the
+// sum over its lower cell (weight a) plus the sum over its upper cell
+// (weight 1-a), each summed from zero in PID order.  dir: 0 = i-faces
+// [KM][JM][IM+1], 1 = j, 2 = k.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_deposit_faces(int dir, int IM, int JM, int KM,
                                                           const int32_t *__restrict__ start,
@@ -589,7 +590,7 @@ __global__ __launch_bounds__(kBlock) void k_deposit_faces(int dir, int IM, int J
   int64_t r = f / fI;
   int j = (int)(r % fJ);
   int k = (int)(r / fJ);
-  FaceAcc<R> a0, a1, a2, a3;
+  FaceAcc<R> acc[2][4];
 #pragma unroll
   for (int side = 0; side < 2; ++side) {
     int ci = i, cj = j, ck = k;
@@ -607,17 +608,15 @@ __global__ __launch_bounds__(kBlock) void k_deposit_faces(int dir, int IM, int J
       last = lp;
       R a = w[3 * q + dir];
       R wt = side == 0 ? a : (R)1 - a;
-      a0.add(wt);
-      a1.add(wt * vel[3 * q]);
-      a2.add(wt * vel[3 * q + 1]);
-      a3.add(wt * vel[3 * q + 2]);
+      acc[side][0].add(wt);
+      acc[side][1].add(wt * vel[3 * q]);
+      acc[side][2].add(wt * vel[3 * q + 1]);
+      acc[side][3].add(wt * vel[3 * q + 2]);
     }
   }
   R *o = out + 4 * f;
-  o[0] = a0.value();
-  o[1] = a1.value();
-  o[2] = a2.value();
-  o[3] = a3.value();
+#pragma unroll
+  for (int v = 0; v < 4; ++v) o[v] = acc[0][v].value() + acc[1][v].value();
 }
 
 // Hexahedron volume as six tetrahedra around the diagonal V0-V5.

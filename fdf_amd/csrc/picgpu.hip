@@ -297,7 +297,7 @@ int picgpu_upload_grid(picgpu_ctx *c, int64_t IM, int64_t JM, int64_t KM, const 
     return PICGPU_E_ARG;
   }
   PG_HIP(hipSetDevice(c->device));
-  PG_CHECK(settle_swarm(c));  // the particles stay; the new grid drops their index
+  if (!c->swarm_invalid) PG_CHECK(settle_swarm(c));  // the particles stay; the new grid drops their index
   if (IM * JM * KM >= (int64_t)INT32_MAX - 2 || (IM + 1) * (JM + 1) * (KM + 1) >= ((int64_t)1 << 40)) {
     set_error("upload_grid: grid too large for 32-bit packed cell indices");
     return PICGPU_E_UNSUPPORTED;
@@ -319,7 +319,7 @@ int picgpu_upload_grid_device(picgpu_ctx *c, int64_t IM, int64_t JM, int64_t KM,
     return PICGPU_E_ARG;
   }
   PG_HIP(hipSetDevice(c->device));
-  PG_CHECK(settle_swarm(c));  // the particles stay; the new grid drops their index
+  if (!c->swarm_invalid) PG_CHECK(settle_swarm(c));  // the particles stay; the new grid drops their index
   if (IM * JM * KM >= (int64_t)INT32_MAX - 2) {
     set_error("upload_grid_device: grid too large for 32-bit packed cell indices");
     return PICGPU_E_UNSUPPORTED;
@@ -365,6 +365,7 @@ int picgpu_init_particles_device(picgpu_ctx *c, int64_t n, const double *d_pos, 
   PG_CHECK(ensure_swarm(c, c->sw, n));
   c->n = n;
   c->sorted = c->keyed = c->key_ordered = c->compact_pending = false;
+  c->swarm_invalid = false;
   if (n == 0) return PICGPU_OK;
   int r = dispatch_real(c, [&](auto *tag) -> int {
     using R = std::remove_pointer_t<decltype(tag)>;
@@ -421,6 +422,7 @@ int picgpu_upload_particles(picgpu_ctx *c, int64_t n, const double *pos, const d
   PG_CHECK(ensure_swarm(c, c->sw, n));
   c->n = n;
   c->sorted = c->keyed = c->key_ordered = c->compact_pending = false;
+  c->swarm_invalid = false;
   if (n == 0) return PICGPU_OK;
   PG_HIP(hipMemcpy(c->sw.pos.ptr, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice));
   PG_HIP(hipMemcpy(c->sw.pid.ptr, pid, sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice));
@@ -891,6 +893,11 @@ static int compact_by_key(picgpu_ctx *c, int64_t live, int mode) {
 }
 
 int picgpu::settle_swarm(picgpu_ctx *c) {
+  if (c->swarm_invalid) {
+    set_error("the particles are invalid after a step that met a degenerate face: upload, initialise or seed "
+              "particles first");
+    return PICGPU_E_STATE;
+  }
   if (!c->compact_pending) return PICGPU_OK;
   c->compact_pending = false;
   PG_CHECK(compact_by_key(c, c->n_live, GATHER_LAST));
@@ -997,10 +1004,10 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool last) {
   unsigned long long h[C_NUM];
   PG_CHECK(counters_read(c, h));
   if (h[C_ERROR]) {
-    // drop the particles earlier steps removed (through the index this step
-    // started from) before the index is given up
-    PG_CHECK(settle_swarm(c));
-    c->keyed = false;
+    // the step stopped part-way: some particles carry this step's results,
+    // the others the state it started from, and the index is half built
+    c->swarm_invalid = true;
+    c->keyed = c->key_ordered = c->compact_pending = false;
     set_error("degenerate cell face (|n| < 1e-12) met during the walk");
     return PICGPU_E_DEGENERATE;
   }
@@ -1124,6 +1131,15 @@ int picgpu_deposit_faces(picgpu_ctx *c, double particle_volume) {
   return PICGPU_OK;
 }
 
+int picgpu_set_particle_volume(picgpu_ctx *c, double particle_volume) {
+  if (!c || !(particle_volume > 0.0 && particle_volume < HUGE_VAL)) {
+    set_error("particle volume must be positive and finite");
+    return PICGPU_E_ARG;
+  }
+  c->particle_volume = particle_volume;
+  return PICGPU_OK;
+}
+
 int picgpu_step(picgpu_ctx *c, double dt, int nsteps, unsigned flags) {
   if (!c || nsteps < 0) return PICGPU_E_ARG;
   PG_HIP(hipSetDevice(c->device));
@@ -1132,6 +1148,7 @@ int picgpu_step(picgpu_ctx *c, double dt, int nsteps, unsigned flags) {
     set_error("ucat not uploaded");
     return PICGPU_E_STATE;
   }
+  if (c->swarm_invalid) return settle_swarm(c);  // reports that the particles are invalid
   if (flags & PICGPU_STEP_STAGED) PG_CHECK(settle_swarm(c));
   if (!c->compact_pending) c->steps_since_compact = 0;
   const bool fields = (flags & PICGPU_STEP_FIELDS) != 0;

@@ -160,6 +160,11 @@ struct picgpu_ctx {
   // particles and the arrays (length n) still hold removed ones; settle_swarm
   // compacts them before anything else reads or changes the arrays
   bool compact_pending = false;
+  // a fused step met a degenerate face part-way: the arrays hold a mix of
+  // that step's results and the state it started from; every entry point that
+  // reads or changes the particles fails (PICGPU_E_STATE) until new particles
+  // are uploaded, initialised or seeded
+  bool swarm_invalid = false;
   int64_t kbegin_unloc = 0, kend_unloc = 0;  // range of bucket nkeys in kidx[kcur]
 
   // per-cell / per-face outputs

@@ -179,6 +179,10 @@ class PicGpu:
     def DepositParticlesToFaces(self, particle_volume: float = 1.0):
         L.check(self.lib.picgpu_deposit_faces(self.h, particle_volume))
 
+    def set_particle_volume(self, particle_volume: float):
+        """V_p of the face volume fractions (count * V_p / V_dual)."""
+        L.check(self.lib.picgpu_set_particle_volume(self.h, particle_volume))
+
     def AdvanceSimulation(self, dt: float, nsteps: int, rk2: bool = False, deposit_cells: bool = True,
                           deposit_faces: bool = False, staged: bool = False, fields: bool = False):
         """fields=True: every step first prepares Ucat (boundaries, Contra2Cart,

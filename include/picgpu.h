@@ -164,6 +164,10 @@ int picgpu_remove_out_of_bounds(picgpu_ctx *ctx, int64_t *removed);
 int picgpu_interpolate(picgpu_ctx *ctx);
 int picgpu_scatter(picgpu_ctx *ctx);
 int picgpu_deposit_faces(picgpu_ctx *ctx, double particle_volume);
+/* Particle volume V_p of the face volume fractions (PICGPU_FIELD_*FACE_VOLFRAC
+ * = count * V_p / V_dual); default 1.  picgpu_deposit_faces sets it too.
+ * V_p <= 0 or not finite: PICGPU_E_ARG. */
+int picgpu_set_particle_volume(picgpu_ctx *ctx, double particle_volume);
 
 /* ---- whole steps: reset, advect(+locate) [Euler|RK2], remove, interpolate, deposit ---- */
 int picgpu_step(picgpu_ctx *ctx, double dt, int nsteps, unsigned flags);

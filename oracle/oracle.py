@@ -25,7 +25,7 @@ def build() -> None:
 
 class _Ctx(C.Structure):
     _fields_ = [("IM", C.c_int64), ("JM", C.c_int64), ("KM", C.c_int64), ("coor", C.c_void_p),
-                ("corner", C.c_void_p), ("max_traversal", C.c_int)]
+                ("corner", C.c_void_p), ("max_traversal", C.c_int), ("bbox", C.c_void_p)]
 
 
 class _Grid(C.Structure):
@@ -79,6 +79,8 @@ class Oracle:
             "ref_count_cells": (None, [I64, I64, I64, I64, P, P]),
             "ref_scatter_P": (None, [I64, I64, I64, I64, P, P, P, P, P]),
             "ref_deposit_faces": (None, [I64, I64, I64, I64, P, P, P, P, P, P, P]),
+            "ref_cell_volume": (D, [P]),
+            "ref_face_volfrac": (None, [I64, I64, I64, P, I, P, D, P]),
             "ref_step_euler": (I64, [C.POINTER(_Ctx), I64, P, P, P, P, P, P, P, D, P]),
             "ref_step_rk2": (I64, [C.POINTER(_Ctx), I64, P, P, P, P, P, P, P, D, P]),
             "ref_face_metrics": (None, [I64, I64, I64, P, P, P, P]),
@@ -188,17 +190,32 @@ class Oracle:
                                    _p(fj), _p(fk))
         return fi, fj, fk
 
+    def cell_volume(self, V):
+        V = np.ascontiguousarray(V, np.float64)
+        return self.lib.ref_cell_volume(_p(V))
+
+    def face_volfrac(self, IM, JM, KM, coor, dir, face, vp):
+        """Volume fraction of every face of direction dir from that direction's
+        face array (count, mx, my, mz per face)."""
+        coor = np.ascontiguousarray(coor, np.float64)
+        face = np.ascontiguousarray(face, np.float64)
+        out = np.zeros(face.size // 4)
+        self.lib.ref_face_volfrac(IM, JM, KM, _p(coor), int(dir), _p(face), float(vp), _p(out))
+        return out
+
     def step(self, IM, JM, KM, coor, corner, state: dict, dt: float, rk2: bool = False,
-             max_traversal=MAX_TRAVERSAL):
-        """One reference step on a state dict (pos, vel, cell, weight, status, pid, P); compacts in place."""
+             max_traversal=MAX_TRAVERSAL, bbox=None):
+        """One reference step on a state dict (pos, vel, cell, weight, status, pid, P); compacts in place.
+        bbox: the padded bounding box (lo, hi) when known, else it is computed from the nodes."""
         coor = np.ascontiguousarray(coor, np.float64)
         corner = np.ascontiguousarray(corner, np.float64)
+        bb = None if bbox is None else np.ascontiguousarray(np.concatenate([bbox[0], bbox[1]]), np.float64)
         s = state
         for k, t in (("pos", np.float64), ("vel", np.float64), ("weight", np.float64), ("P", np.float64),
                      ("cell", np.int64), ("status", np.int64), ("pid", np.int64)):
             s[k] = np.ascontiguousarray(s[k], t)
         n = s["pos"].shape[0]
-        ctx = _Ctx(IM, JM, KM, _p(coor), _p(corner), max_traversal)
+        ctx = _Ctx(IM, JM, KM, _p(coor), _p(corner), max_traversal, _p(bb))
         removed = C.c_int64()
         fn = self.lib.ref_step_rk2 if rk2 else self.lib.ref_step_euler
         m = fn(C.byref(ctx), n, _p(s["pos"]), _p(s["vel"]), _p(s["cell"]), _p(s["weight"]), _p(s["status"]),

@@ -402,48 +402,56 @@ void ref_scatter_P(int64_t IM, int64_t JM, int64_t KM, int64_t n, const int64_t 
 }
 
 /* ---- SPEC: face deposition ---- */
-typedef struct {
-  int64_t key, pid, p;
-} ref_kp_t;
-static int cmp_kp(const void *a, const void *b) {
-  const ref_kp_t *x = (const ref_kp_t *)a, *y = (const ref_kp_t *)b;
-  if (x->key != y->key) return x->key < y->key ? -1 : 1;
-  if (x->pid != y->pid) return x->pid < y->pid ? -1 : 1;
-  return 0;
+/* The particles of every cell in ascending PID order: a counting sort by cell
+ * (stable in particle order), then an insertion sort of each cell's bucket by
+ * PID.  out[start[c] .. start[c+1]) lists cell c's particles. */
+static int64_t *cells_in_pid_order(int64_t IM, int64_t JM, int64_t KM, int64_t n, const int64_t *cell,
+                                   const int64_t *pid, int64_t **start_out) {
+  const int64_t ncell = IM * JM * KM;
+  int64_t *start = (int64_t *)calloc((size_t)(ncell + 2), sizeof(int64_t));
+  int64_t *list = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+  for (int64_t p = 0; p < n; ++p) {
+    int64_t i = cell[3 * p], j = cell[3 * p + 1], k = cell[3 * p + 2];
+    if (i < 0 || i >= IM || j < 0 || j >= JM || k < 0 || k >= KM) continue;
+    start[(k * JM + j) * IM + i + 2]++;
+  }
+  for (int64_t c = 0; c < ncell; ++c) start[c + 2] += start[c + 1];
+  for (int64_t p = 0; p < n; ++p) {
+    int64_t i = cell[3 * p], j = cell[3 * p + 1], k = cell[3 * p + 2];
+    if (i < 0 || i >= IM || j < 0 || j >= JM || k < 0 || k >= KM) continue;
+    list[start[(k * JM + j) * IM + i + 1]++] = p;
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < ncell; ++c)
+    for (int64_t t = start[c] + 1; t < start[c + 1]; ++t) {
+      const int64_t x = list[t];
+      int64_t u = t - 1;
+      while (u >= start[c] && pid[list[u]] > pid[x]) {
+        list[u + 1] = list[u];
+        --u;
+      }
+      list[u + 1] = x;
+    }
+  *start_out = start;
+  return list;
 }
 
 void ref_deposit_faces(int64_t IM, int64_t JM, int64_t KM, int64_t n, const int64_t *cell,
                        const double *w, const double *vel, const int64_t *pid, double *iface,
                        double *jface, double *kface) {
-  const int64_t ncell = IM * JM * KM;
-  /* sort particles by (cell, pid): the order every face sum uses */
-  ref_kp_t *kp = (ref_kp_t *)malloc(sizeof(ref_kp_t) * (size_t)(n > 0 ? n : 1));
-  int64_t m = 0;
-  for (int64_t p = 0; p < n; ++p) {
-    int64_t i = cell[3 * p], j = cell[3 * p + 1], k = cell[3 * p + 2];
-    if (i < 0 || i >= IM || j < 0 || j >= JM || k < 0 || k >= KM) continue;
-    kp[m].key = (k * JM + j) * IM + i;
-    kp[m].pid = pid[p];
-    kp[m].p = p;
-    m++;
-  }
-  qsort(kp, (size_t)m, sizeof(ref_kp_t), cmp_kp);
-  int64_t *start = (int64_t *)calloc((size_t)(ncell + 1), sizeof(int64_t));
-  for (int64_t q = 0; q < m; ++q) start[kp[q].key + 1]++;
-  for (int64_t c = 0; c < ncell; ++c) start[c + 1] += start[c];
-  const int64_t nfi = KM * JM * (IM + 1), nfj = KM * (JM + 1) * IM, nfk = (KM + 1) * JM * IM;
-  memset(iface, 0, sizeof(double) * 4 * nfi);
-  memset(jface, 0, sizeof(double) * 4 * nfj);
-  memset(kface, 0, sizeof(double) * 4 * nfk);
-  /* face f between lower cell cl and upper cell cu: sum over cl (weight a),
-   * then over cu (weight 1-a), each in PID order */
+  int64_t *start;
+  int64_t *list = cells_in_pid_order(IM, JM, KM, n, cell, pid, &start);
+  /* face f between lower cell cl and upper cell cu: the sum over cl's
+   * particles (weight a) and the sum over cu's (weight 1-a), each from zero
+   * in PID order, then lower + upper; a missing cell adds zero */
   for (int dir = 0; dir < 3; ++dir) {
-    int64_t fI = IM + (dir == 0), fJ = JM + (dir == 1), fK = KM + (dir == 2);
+    const int64_t fI = IM + (dir == 0), fJ = JM + (dir == 1), fK = KM + (dir == 2);
     double *out = dir == 0 ? iface : (dir == 1 ? jface : kface);
+#pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < fK; ++k)
       for (int64_t j = 0; j < fJ; ++j)
         for (int64_t i = 0; i < fI; ++i) {
-          double acc[4] = {0.0, 0.0, 0.0, 0.0};
+          double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
           for (int side = 0; side < 2; ++side) {
             int64_t ci = i, cj = j, ck = k;
             if (side == 0) { /* lower cell */
@@ -452,26 +460,66 @@ void ref_deposit_faces(int64_t IM, int64_t JM, int64_t KM, int64_t n, const int6
               if (dir == 2) ck -= 1;
             }
             if (ci < 0 || ci >= IM || cj < 0 || cj >= JM || ck < 0 || ck >= KM) continue;
-            int64_t c = (ck * JM + cj) * IM + ci;
+            const int64_t c = (ck * JM + cj) * IM + ci;
+            double *s = acc[side];
             for (int64_t q = start[c]; q < start[c + 1]; ++q) {
-              int64_t p = kp[q].p;
-              double a = w[3 * p + dir];
-              double wt = side == 0 ? a : 1.0 - a;
-              acc[0] += wt;
-              acc[1] += wt * vel[3 * p];
-              acc[2] += wt * vel[3 * p + 1];
-              acc[3] += wt * vel[3 * p + 2];
+              const int64_t p = list[q];
+              const double a = w[3 * p + dir];
+              const double wt = side == 0 ? a : 1.0 - a;
+              s[0] += wt;
+              s[1] += wt * vel[3 * p];
+              s[2] += wt * vel[3 * p + 1];
+              s[3] += wt * vel[3 * p + 2];
             }
           }
           double *o = out + 4 * ((k * fJ + j) * fI + i);
-          o[0] = acc[0];
-          o[1] = acc[1];
-          o[2] = acc[2];
-          o[3] = acc[3];
+          for (int v = 0; v < 4; ++v) o[v] = acc[0][v] + acc[1][v];
         }
   }
   free(start);
-  free(kp);
+  free(list);
+}
+
+/* SPEC: hexahedron volume as six tetrahedra around the diagonal V0-V5 (vertex
+ * order walkingsearch.c:537-544); the tetrahedra (V0, Va, Vb, V5) for the
+ * ring edges (a, b) in the order below, |det| summed, divided by 6. */
+double ref_cell_volume(const double V[8][3]) {
+  static const int T[6][2] = {{1, 2}, {1, 6}, {3, 2}, {3, 4}, {7, 6}, {7, 4}};
+  double s = 0.0;
+  for (int t = 0; t < 6; ++t) {
+    const double *a = V[0], *b = V[T[t][0]], *c = V[T[t][1]], *d = V[5];
+    const double x0 = b[0] - a[0], x1 = b[1] - a[1], x2 = b[2] - a[2];
+    const double y0 = c[0] - a[0], y1 = c[1] - a[1], y2 = c[2] - a[2];
+    const double z0 = d[0] - a[0], z1 = d[1] - a[1], z2 = d[2] - a[2];
+    const double det = x0 * (y1 * z2 - y2 * z1) + x1 * (y2 * z0 - y0 * z2) + x2 * (y0 * z1 - y1 * z0);
+    s += fabs(det);
+  }
+  return s / 6.0;
+}
+
+/* SPEC: face volume fraction = count * vp / V_dual, V_dual = half the summed
+ * volumes of the face's cells (lower cell first; a boundary face has one). */
+void ref_face_volfrac(int64_t IM, int64_t JM, int64_t KM, const double *coor, int dir, const double *face,
+                      double vp, double *out) {
+  const ref_grid_t g = {IM, JM, KM, coor};
+  const int64_t fI = IM + (dir == 0), fJ = JM + (dir == 1), fK = KM + (dir == 2);
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < fK; ++k)
+    for (int64_t j = 0; j < fJ; ++j)
+      for (int64_t i = 0; i < fI; ++i) {
+        double V[8][3], s = 0.0;
+        const int64_t li = i - (dir == 0), lj = j - (dir == 1), lk = k - (dir == 2);
+        if (li >= 0 && lj >= 0 && lk >= 0) {
+          ref_cell_vertices(&g, li, lj, lk, V);
+          s += ref_cell_volume(V);
+        }
+        if (i < IM && j < JM && k < KM) {
+          ref_cell_vertices(&g, i, j, k, V);
+          s += ref_cell_volume(V);
+        }
+        const int64_t f = (k * fJ + j) * fI + i;
+        out[f] = (face[4 * f] * vp) / (0.5 * s);
+      }
 }
 
 /* ---- step drivers ---- */
@@ -479,7 +527,14 @@ static int64_t finish_step(const ref_ctx_t *c, int64_t n, double *pos, double *v
                            double *w, int64_t *status, int64_t *pid, double *P, int64_t *removed) {
   ref_grid_t g = {c->IM, c->JM, c->KM, c->coor};
   double bmin[3], bmax[3];
-  ref_bbox(&g, bmin, bmax);
+  if (c->bbox) {
+    for (int a = 0; a < 3; ++a) {
+      bmin[a] = c->bbox[a];
+      bmax[a] = c->bbox[3 + a];
+    }
+  } else {
+    ref_bbox(&g, bmin, bmax);
+  }
   int64_t rl = 0, ro = 0;
   n = ref_remove(n, pos, vel, cell, w, status, pid, P, bmin, bmax, &rl, &ro);
   if (removed) *removed = rl + ro;

@@ -101,13 +101,24 @@ void ref_scatter_P(int64_t IM, int64_t JM, int64_t KM, int64_t n, const int64_t 
                    const double *P, const int64_t *order, const double *count, double *avg);
 /* SPEC: face deposition.  Particle in cell (i,j,k) with weights (a1,a2,a3)
  * adds (1-a1) to i-face i and a1 to i-face i+1 (likewise j, k).  Outputs
- * count and momentum (3) per face.  Contributions to each face are summed
- * in ascending PID order, from the lower cell first, then the upper cell.
+ * count and momentum (3) per face.  A face is the sum of its lower cell's
+ * contributions plus the sum of its upper cell's, each summed from zero in
+ * ascending PID order (a missing cell contributes zero).
  * Layouts: iface [KM][JM][IM+1], jface [KM][JM+1][IM], kface [KM+1][JM][IM],
  * each entry 4 doubles (count, mx, my, mz). */
 void ref_deposit_faces(int64_t IM, int64_t JM, int64_t KM, int64_t n, const int64_t *cell,
                        const double *w, const double *vel, const int64_t *pid,
                        double *iface, double *jface, double *kface);
+/* SPEC: cell volume, six tetrahedra (V0, Va, Vb, V5) around the V0-V5
+ * diagonal for the ring edges (1,2) (1,6) (3,2) (3,4) (7,6) (7,4), |det|
+ * summed in that order, divided by 6. */
+double ref_cell_volume(const double V[8][3]);
+/* SPEC: face volume fraction of direction dir (0 i, 1 j, 2 k) from that
+ * direction's face array (4 doubles per face, count first):
+ * count * vp / V_dual, V_dual = 0.5 * (volume of the lower cell + volume of
+ * the upper cell), lower first, a boundary face having one cell. */
+void ref_face_volfrac(int64_t IM, int64_t JM, int64_t KM, const double *coor, int dir, const double *face,
+                      double vp, double *out);
 
 /* ---- full step drivers (AdvanceSimulation_TEST stage order, simulation.c:248-292) ---- */
 typedef struct {
@@ -115,6 +126,8 @@ typedef struct {
   const double *coor;        /* node coordinates */
   const double *corner;      /* centre->corner result, node slots */
   int max_traversal;
+  const double *bbox;        /* padded bounding box (min xyz, max xyz), or NULL: computed
+                                from the nodes at every step (ref_bbox) */
 } ref_ctx_t;
 
 /* One reference step with forward Euler.  Arrays are compacted in place.

@@ -1,6 +1,6 @@
 """Writes the golden fixtures of this directory from the CPU oracle.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fixture ...]
 
 The fixtures are small seeded cases (inputs and the oracle's outputs) that
 pin the oracle against regressions and give the GPU tests fixed expected
@@ -55,10 +55,14 @@ def walk_warped_small(o: Oracle) -> None:
           status=st, steps=steps, start=start, cell2=cell2, weight2=w2, status2=st2, steps2=steps2)
 
 
+VP = 1.0e-3  # particle volume of the face volume fractions
+
+
 def interp_and_deposit_small(o: Oracle) -> None:
     """On the same warped grid: a random ucat -> corner velocities, trilinear
-    interpolation at the located particles, cell counts, P average and the
-    face deposits (count + momentum)."""
+    interpolation at the located particles, cell counts, P average, the face
+    deposits (count + momentum) and the face volume fractions (particle
+    volume VP)."""
     g = np.load(os.path.join(HERE, "walk_warped_small.npz"))
     IM, JM, KM = (int(v) for v in g["dims"])
     ucat = np.zeros(((KM + 1) * (JM + 1) * (IM + 1), 3))
@@ -73,8 +77,9 @@ def interp_and_deposit_small(o: Oracle) -> None:
     P = np.full(len(cell), 101325.0)
     pavg = o.scatter_P(IM, JM, KM, cell, P, pid, count)
     fi, fj, fk = o.deposit_faces(IM, JM, KM, cell, w, vel, pid)
+    vf = [o.face_volfrac(IM, JM, KM, g["coor"], d, f, VP) for d, f in enumerate((fi, fj, fk))]
     _save("interp_deposit_warped_small.npz", ucat=ucat, corner=corner, vel=vel, pid=pid, count=count, pavg=pavg,
-          fi=fi, fj=fj, fk=fk)
+          fi=fi, fj=fj, fk=fk, vp=np.array(VP), vfi=vf[0], vfj=vf[1], vfk=vf[2])
 
 
 def steps_rk2_small(o: Oracle) -> None:
@@ -111,13 +116,19 @@ def mlp_small(o: Oracle) -> None:
           grad_fp64=grad0)
 
 
+FIXTURES = {"walk_uniform_cfg1": walk_uniform_cfg1, "walk_warped_small": walk_warped_small,
+            "interp_deposit_warped_small": interp_and_deposit_small, "steps_rk2_warped_small": steps_rk2_small,
+            "mlp_small": mlp_small}
+
+
 def main() -> None:
+    """All fixtures, or the ones named on the command line (later fixtures
+    read the walk fixture, so it is written first when named)."""
     o = Oracle()
-    walk_uniform_cfg1(o)
-    walk_warped_small(o)
-    interp_and_deposit_small(o)
-    steps_rk2_small(o)
-    mlp_small(o)
+    names = sys.argv[1:] or list(FIXTURES)
+    for name in FIXTURES:
+        if name in names:
+            FIXTURES[name](o)
 
 
 if __name__ == "__main__":

@@ -86,6 +86,13 @@ def test_interp_and_deposit_small(oracle):
     nloc = (g["status"] == 1).sum()
     for a in faces:
         assert abs(a.reshape(-1, 4)[:, 0].sum() - nloc) < 1e-9 * nloc
+    vp = float(f["vp"])
+    for d, (a, name) in enumerate(zip(faces, ("vfi", "vfj", "vfk"))):
+        vf = oracle.face_volfrac(IM, JM, KM, g["coor"], d, a, vp)
+        assert np.array_equal(vf, f[name])
+        # volume fraction x dual volume returns count x vp: summed over a
+        # direction's faces, the dual volumes weight each particle's count
+        assert (vf >= 0).all() and (vf[a.reshape(-1, 4)[:, 0] == 0] == 0).all()
 
 
 def test_steps_rk2_small(oracle):

@@ -191,15 +191,18 @@ def test_scatter_and_faces(swarm_cls, oracle, precision):
     cnt = oracle.count_cells(IM, JM, KM, cell)
     pav = oracle.scatter_P(IM, JM, KM, cell, P, pid, cnt)
     fi, fj, fk = oracle.deposit_faces(IM, JM, KM, cell, w, vel, pid)
+    vp = 2.0e-3
+    vf = [oracle.face_volfrac(IM, JM, KM, coor, d, f, vp) for d, f in enumerate((fi, fj, fk))]
     with swarm_cls(precision=precision) as g:
         g.upload_grid(IM, JM, KM, coor)
         g.upload_particles(pos, vel=vel, cell=cell, weight=w, status=st, pid=pid, P=P)
         g.ScatterAllParticleFieldsToEulerFields()
-        g.DepositParticlesToFaces()
+        g.DepositParticlesToFaces(particle_volume=vp)
         gcnt = g.field(L.FIELD_PARTICLE_COUNT)
         gpav = g.field(L.FIELD_P_AVERAGE)
         gf = [g.field(L.FIELD_IFACE), g.field(L.FIELD_JFACE), g.field(L.FIELD_KFACE)]
         gf2 = [g.field(L.FIELD_IFACE), g.field(L.FIELD_JFACE), g.field(L.FIELD_KFACE)]
+        gvf = [g.field(f) for f in (L.FIELD_IFACE_VOLFRAC, L.FIELD_JFACE_VOLFRAC, L.FIELD_KFACE_VOLFRAC)]
         n = g.n
     assert n == 30000
     assert np.array_equal(gcnt, cnt)
@@ -207,12 +210,19 @@ def test_scatter_and_faces(swarm_cls, oracle, precision):
         assert np.array_equal(gpav, pav)
         for a, b in zip(gf, (fi, fj, fk)):
             assert np.array_equal(a, b)
+        for a, b in zip(gvf, vf):
+            assert np.array_equal(a, b)
     else:
         assert np.abs(gpav - pav).max() <= FP32_TOL * 101325.0
         for a, b in zip(gf, (fi, fj, fk)):
             a4, b4 = a.reshape(-1, 4), b.reshape(-1, 4)
             assert np.abs(a4[:, 0] - b4[:, 0]).max() <= FP32_TOL * max(1.0, np.abs(b4[:, 0]).max())
             assert np.abs(a4[:, 1:] - b4[:, 1:]).max() <= FP32_TOL * max(1.0, np.abs(b4[:, 1:]).max())
+        for a, b in zip(gvf, vf):
+            assert np.abs(a - b).max() <= FP32_TOL * np.abs(b).max()
+        # the volume fraction of the FP32 counts themselves is the oracle's arithmetic, bit for bit
+        for d, (a, f) in enumerate(zip(gvf, gf)):
+            assert np.array_equal(a, oracle.face_volfrac(IM, JM, KM, coor, d, f, vp))
     for a, b in zip(gf, gf2):
         assert np.array_equal(a, b)
 
@@ -251,10 +261,12 @@ def test_full_steps_against_oracle(swarm_cls, oracle, case, precision, path):
         nsteps, rk2 = 5, True
     IM, JM, KM = d["IM"], d["JM"], d["KM"]
     s, corner = _run_oracle_steps(oracle, d, nsteps, rk2)
+    vp = 1.0 / (IM * JM * KM)  # particle volume of the volume fractions (config 2's output)
     with swarm_cls(precision=precision) as g:
         g.upload_grid(IM, JM, KM, d["coor"])
         g.upload_ucat(d["ucat"])
         g.upload_particles(d["pos"])
+        g.set_particle_volume(vp)
         g.LocateAllParticlesInGrid()
         g.InterpolateAllFieldsToSwarm()
         # split the steps over two calls: the fused path compacts between calls
@@ -266,6 +278,10 @@ def test_full_steps_against_oracle(swarm_cls, oracle, case, precision, path):
         cnt = g.field(L.FIELD_PARTICLE_COUNT)
         pav = g.field(L.FIELD_P_AVERAGE)
         faces = [g.field(L.FIELD_IFACE), g.field(L.FIELD_JFACE), g.field(L.FIELD_KFACE)]
+        vfs = [g.field(f) for f in (L.FIELD_IFACE_VOLFRAC, L.FIELD_JFACE_VOLFRAC, L.FIELD_KFACE_VOLFRAC)]
+    for dd, (a, f) in enumerate(zip(vfs, faces)):
+        # the volume fraction of the deposited counts, bit for bit the oracle's arithmetic
+        assert np.array_equal(a, oracle.face_volfrac(IM, JM, KM, d["coor"], dd, f, vp))
     assert set(np.unique(pav)) <= {0.0, 101325.0} and ((pav > 0) == (cnt > 0)).all()
     if case == "config1" and nsteps == 10:
         # z moves by 10 x 0.1: every particle leaves the unit cube (simulation.c:263-265 removes them)
@@ -281,6 +297,8 @@ def test_full_steps_against_oracle(swarm_cls, oracle, case, precision, path):
         ref = oracle.deposit_faces(IM, JM, KM, s["cell"], s["weight"], s["vel"], s["pid"])
         for a, b in zip(faces, ref):
             assert np.array_equal(a, b)
+        for dd, (a, f) in enumerate(zip(vfs, ref)):
+            assert np.array_equal(a, oracle.face_volfrac(IM, JM, KM, d["coor"], dd, f, vp))
     else:
         # FP32 trajectories drift from FP64 ones; ownership is exact on the
         # variant's own positions, and the drift is small
@@ -297,6 +315,12 @@ def test_full_steps_against_oracle(swarm_cls, oracle, case, precision, path):
         v2 = oracle.interp(IM, JM, KM, corner, c2, w2)
         assert np.abs(v2 - out["vel"]).max() <= FP32_TOL * np.abs(d["ucat"]).max()
         assert np.array_equal(cnt, oracle.count_cells(IM, JM, KM, out["cell"]))
+        # faces of the FP32 run's own state against the FP64 oracle's sums of it
+        ref = oracle.deposit_faces(IM, JM, KM, out["cell"], out["weight"], out["vel"], out["pid"])
+        for a, b in zip(faces, ref):
+            a4, b4 = a.reshape(-1, 4), b.reshape(-1, 4)
+            assert np.abs(a4[:, 0] - b4[:, 0]).max() <= FP32_TOL * max(1.0, np.abs(b4[:, 0]).max())
+            assert np.abs(a4[:, 1:] - b4[:, 1:]).max() <= FP32_TOL * max(1.0, np.abs(b4[:, 1:]).max())
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
