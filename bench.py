@@ -392,43 +392,119 @@ def roofline_for(res: dict, d: dict, g: dict) -> dict:
             "share_of_step_time": kern[dom]["total_ms"] / (1e3 * res["seconds"])}
 
 
-def cpu_baseline(d: dict, sample_particles: int, sample_steps: int) -> dict:
-    """The FP64 oracle built -O3 with OpenMP, on a bounded sample of the same
-    workload (full grid and field, a subset of the particles)."""
+def _host_cpus() -> dict:
+    """CPUs this process may run on, the machine's count and the CPU model
+    (what lscpu prints as "Model name", read from /proc/cpuinfo)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"nproc": avail, "cpu_count": os.cpu_count(), "model": model}
+
+
+def _sub_block(d: dict, dims) -> dict:
+    """A block of ni x nj x nk cells from the middle of the workload's grid, its
+    field and the workload's particles inside the block's lattice box (the
+    same density as the whole workload), all on the host."""
+    import torch
+
+    IM, JM, KM = d["IM"], d["JM"], d["KM"]
+    ni, nj, nk = (min(a, b) for a, b in zip(dims, (IM, JM, KM)))
+    i0, j0, k0 = (IM - ni) // 2, (JM - nj) // 2, (KM - nk) // 2
+    c = d["coor"].reshape(KM + 1, JM + 1, IM + 1, 3)[k0:k0 + nk + 1, j0:j0 + nj + 1, i0:i0 + ni + 1]
+    u = torch.zeros((nk + 1, nj + 1, ni + 1, 3), dtype=torch.float64, device=d["ucat"].device)
+    u[:nk, :nj, :ni] = d["ucat"].reshape(KM + 1, JM + 1, IM + 1, 3)[k0:k0 + nk, j0:j0 + nj, i0:i0 + ni]
+    lo = torch.tensor([i0 / IM, j0 / JM, k0 / KM], dtype=torch.float64, device=d["pos_dev"].device)
+    hi = torch.tensor([(i0 + ni) / IM, (j0 + nj) / JM, (k0 + nk) / KM], dtype=torch.float64,
+                      device=d["pos_dev"].device)
+    inside = ((d["pos_dev"] >= lo) & (d["pos_dev"] < hi)).all(1)
+    return {"IM": ni, "JM": nj, "KM": nk, "origin": (i0, j0, k0), "coor": c.contiguous().cpu().numpy().reshape(-1, 3),
+            "ucat": u.cpu().numpy().reshape(-1, 3), "pos": d["pos_dev"][inside].cpu().numpy()}
+
+
+def _cpu_rate(o, blk: dict, d: dict, threads: int, runs: int, steps: int) -> dict:
+    """Median over `runs` runs of the oracle's per-step loop on a sub-block:
+    one untimed warm-up step, then `steps` timed steps, each the workload's
+    step (centre->corner, reset, advection with the walking-search locate,
+    removal, interpolation, deposition)."""
+    import ctypes
+
     import numpy as np
 
-    from oracle.oracle import Oracle
-
-    threads = min(16, os.cpu_count() or 1)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
-    o = Oracle(fast=True)
-    IM, JM, KM = d["IM"], d["JM"], d["KM"]
-    coor = d["coor"].cpu().numpy().reshape(-1, 3)
-    ucat = d["ucat"].cpu().numpy().reshape(-1, 3)
-    pos = d["pos_dev"][:sample_particles].cpu().numpy()
-    n = pos.shape[0]
-    corner = o.center_to_corner(IM, JM, KM, ucat)
-    # start from located particles (the initial setup is not timed)
-    cell, w, st, _ = o.locate(IM, JM, KM, coor, pos)
-    s = {"pos": pos.copy(), "vel": o.interp(IM, JM, KM, corner, cell, w), "cell": cell, "weight": w,
-         "status": st, "pid": np.arange(n, dtype=np.int64), "P": np.full(n, 101325.0)}
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(threads))
+    IM, JM, KM, coor = blk["IM"], blk["JM"], blk["KM"], blk["coor"]
+    bbox = o.bbox(IM, JM, KM, coor)
+    corner = o.center_to_corner(IM, JM, KM, blk["ucat"])
+    n = blk["pos"].shape[0]
+    cell, w, st, _ = o.locate(IM, JM, KM, coor, blk["pos"])
+    s0 = {"pos": blk["pos"].copy(), "vel": o.interp(IM, JM, KM, corner, cell, w), "cell": cell, "weight": w,
+          "status": st, "pid": np.arange(n, dtype=np.int64), "P": np.full(n, 101325.0)}
     faces = "faces" in d["deposit"]
-    t0 = time.perf_counter()
-    ps = 0
-    for _ in range(sample_steps):
-        ps += s["pos"].shape[0]
-        corner = o.center_to_corner(IM, JM, KM, ucat)
-        o.step(IM, JM, KM, coor, corner, s, d["dt"], rk2=d["rk2"])
+
+    def one_step(s):
+        c2 = o.center_to_corner(IM, JM, KM, blk["ucat"])
+        o.step(IM, JM, KM, coor, c2, s, d["dt"], rk2=d["rk2"], bbox=bbox)
         if faces:
             o.deposit_faces(IM, JM, KM, s["cell"], s["weight"], s["vel"], s["pid"])
         else:
-            o.count_cells(IM, JM, KM, s["cell"])
-    dt = time.perf_counter() - t0
-    return {"value": ps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"config {d['config']} full grid {IM}x{JM}x{KM}, {n} of {d['n']} particles, "
-                      f"{sample_steps} steps ({'RK2' if d['rk2'] else 'Euler'}, "
-                      f"{'face' if faces else 'cell'} deposition, centre->corner each step), "
-                      f"FP64 oracle -O3 x86-64-v3 OpenMP, {dt:.1f} s"}
+            cnt = o.count_cells(IM, JM, KM, s["cell"])
+            o.scatter_P(IM, JM, KM, s["cell"], s["P"], s["pid"], cnt)
+
+    rates, secs = [], []
+    for _ in range(runs):
+        s = {k: v.copy() for k, v in s0.items()}
+        one_step(s)  # warm-up
+        ps = 0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ps += s["pos"].shape[0]
+            one_step(s)
+        dt = time.perf_counter() - t0
+        rates.append(ps / dt)
+        secs.append(dt)
+    return {"value": float(np.median(rates)), "runs": [float(r) for r in rates], "seconds": float(sum(secs)),
+            "threads": int(threads), "cells": [IM, JM, KM], "particles": int(n)}
+
+
+def cpu_baseline(d: dict, all_block=(128, 128, 64), one_block=(128, 128, 64), runs: int = 3,
+                 steps: int = 3) -> dict:
+    """The FP64 oracle built -O3 with OpenMP, timed on the host cores on a
+    bounded sample of the same workload: a block of cells from the middle of
+    the grid with the workload's particles inside it (same density, field,
+    dt, advection and deposition).  Grid work and particle work keep their
+    full-size proportions, so the rate (particle-steps/s) is the full
+    workload's on these cores.  Median of `runs` runs of `steps` steps after
+    a warm-up step, with every available core and with one."""
+    from oracle.oracle import Oracle
+
+    host = _host_cpus()
+    threads = max(1, min(16, host["nproc"]))
+    o = Oracle(fast=True)
+    blk_all = _sub_block(d, all_block)
+    blk_one = _sub_block(d, one_block) if tuple(one_block) != tuple(all_block) else blk_all
+    r_all = _cpu_rate(o, blk_all, d, threads, runs, steps)
+    r_one = _cpu_rate(o, blk_one, d, 1, runs, steps)
+
+    def desc(r):
+        return (f"{r['cells'][0]}x{r['cells'][1]}x{r['cells'][2]} cells from the middle of the "
+                f"{d['IM']}x{d['JM']}x{d['KM']} grid with its {r['particles']} particles")
+
+    return {"value": r_all["value"], "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"config {d['config']} ({'RK2' if d['rk2'] else 'Euler'}, "
+                      f"{'face' if 'faces' in d['deposit'] else 'cell'} deposition, centre->corner each step): "
+                      f"{desc(r_all)}; median of {runs} runs of {steps} steps after a warm-up step; "
+                      f"FP64 oracle -O3 x86-64-v3 OpenMP; omits the reference's PETSc view overheads",
+            "one_core": {"value": r_one["value"], "cores": 1, "sample": desc(r_one), "runs": r_one["runs"]},
+            "runs": r_all["runs"], "seconds": r_all["seconds"] + r_one["seconds"], "host": host}
 
 
 def fields_cpu_baseline(d: dict, kplanes: int = 64, reps: int = 3) -> dict:
@@ -473,8 +549,8 @@ def main():
     ap.add_argument("--config", type=int, default=5)
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--variants", default="fp32_mlp,fp64")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 19)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-block", default="128,128,64",
+                    help="cells of the CPU baseline's sub-block (timed with every core and with one)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--error-sample", type=int, default=1 << 22,
                     help="particles of the FP32-vs-FP64 one-step error sample (0: skip)")
@@ -551,7 +627,8 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(d, min(args.cpu_sample, d["n"]), args.cpu_steps)
+            blk = tuple(int(v) for v in args.cpu_block.split(","))
+            cpu = cpu_baseline(d, blk, blk)
             log(f"cpu baseline: {cpu['value']:.4g} particle-steps/s on {cpu['cores']} cores")
         except Exception as e:
             log("cpu baseline failed:", e)
