@@ -116,6 +116,7 @@ _SIG = {
     "picgpu_deposit_faces": (C.c_int, [_P, C.c_double]),
     "picgpu_set_particle_volume": (C.c_int, [_P, C.c_double]),
     "picgpu_step": (C.c_int, [_P, C.c_double, C.c_int, C.c_uint]),
+    "picgpu_locate_device_noalloc": (C.c_int, [_P, _I64, _P, _P, _P, _P, C.POINTER(PicgpuStats)]),
     "picgpu_grid_field_size": (C.c_int, [_P, C.c_int, C.POINTER(_I64)]),
     "picgpu_download_grid_field": (C.c_int, [_P, C.c_int, _P]),
     "picgpu_get_stats": (C.c_int, [_P, C.POINTER(PicgpuStats)]),

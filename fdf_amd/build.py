@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpicgpu.so")
-SOURCES = ["picgpu.hip", "scan.hip", "mlp.hip", "eulerian.hip", "io.hip", "seed.hip"]
+SOURCES = ["picgpu.hip", "scan.hip", "mlp.hip", "eulerian.hip", "io.hip", "seed.hip", "noalloc.hip"]
 HEADERS = ["picgpu_internal.hpp", "kernels.hpp", "walk.hpp"]
 
 HIPCC_FLAGS = [

@@ -4,6 +4,8 @@
 // 64-wide wavefronts.  All FP64 arithmetic that must match the reference is
 // written in the reference's operation order; the build uses
 // -ffp-contract=off, and FP32 code asks for fused multiply-adds explicitly.
+// More than one source file includes this header: its kernels are templates
+// or static.
 #pragma once
 #include "picgpu_internal.hpp"
 
@@ -408,7 +410,7 @@ __global__ __launch_bounds__(kBlock) void k_interp(int64_t n, int IM, int JM, in
 // Removal flags (ParticleMotion.c:532-625 LOST; :283-383 bounding box with
 // IsParticleInBox :106-110).  keep[p] = 1 for survivors.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_keep_flags(int64_t n, const double *__restrict__ pos,
+static __global__ __launch_bounds__(kBlock) void k_keep_flags(int64_t n, const double *__restrict__ pos,
                                                        const int32_t *__restrict__ status, int check_lost,
                                                        int check_box, double bx0, double by0, double bz0,
                                                        double bx1, double by1, double bz1,
@@ -470,7 +472,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(int64_t n, const int32_t *__
 // dropped.  The slot order inside a cell is not deterministic; every
 // consumer of a cell segment visits it in ascending PID order.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_cell_histogram(int64_t n, const int32_t *__restrict__ cell,
+static __global__ __launch_bounds__(kBlock) void k_cell_histogram(int64_t n, const int32_t *__restrict__ cell,
                                                            int32_t *__restrict__ count, int32_t *__restrict__ rank) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -662,7 +664,7 @@ __global__ __launch_bounds__(kBlock) void k_face_volfrac(int dir, GridView g, co
 // Device-side input preparation.
 // ---------------------------------------------------------------------------
 // per-block min/max of node coordinates: out[6*b .. 6*b+5] = (min xyz, max xyz)
-__global__ __launch_bounds__(kBlock) void k_bbox_partial(int64_t nnode, const double *__restrict__ coor,
+static __global__ __launch_bounds__(kBlock) void k_bbox_partial(int64_t nnode, const double *__restrict__ coor,
                                                          double *__restrict__ out) {
   __shared__ double sh[6][kBlock];
   double lo[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
@@ -733,7 +735,7 @@ __global__ __launch_bounds__(kBlock) void k_init_particles(int64_t n, const doub
   P[p] = (R)101325.0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_keep_unlocated(int64_t n, const int32_t *__restrict__ cell,
+static __global__ __launch_bounds__(kBlock) void k_keep_unlocated(int64_t n, const int32_t *__restrict__ cell,
                                                            int32_t *__restrict__ keep) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p < n) keep[p] = cell[p] < 0 ? 1 : 0;

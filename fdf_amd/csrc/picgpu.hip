@@ -246,6 +246,12 @@ int picgpu_create(picgpu_ctx **out, const picgpu_options *opt) {
     set_error("HIP stream creation failed");
     return PICGPU_E_HIP;
   }
+  const int rs = noalloc_setup(c->device);
+  if (rs != PICGPU_OK) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rs;
+  }
   *out = c;
   return PICGPU_OK;
 }

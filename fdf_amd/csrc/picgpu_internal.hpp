@@ -229,6 +229,10 @@ int eulerian_fields_end(picgpu_ctx *c);
 // node-slot Ucat; kind: hipMemcpyHostToDevice or hipMemcpyDeviceToDevice
 int eulerian_take_ucat(picgpu_ctx *c, const double *ucat, hipMemcpyKind kind);
 
+// the device address of picgpu_locate_device_noalloc's scratch array on
+// `device` (the current device), looked up once per device (noalloc.hip)
+int noalloc_setup(int device);
+
 // MLP
 int mlp_guess_device(picgpu_ctx *c, int64_t n, const double *d_pos, const int32_t *d_cell_or_null,
                      int32_t *d_guess);

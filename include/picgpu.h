@@ -169,6 +169,31 @@ int picgpu_deposit_faces(picgpu_ctx *ctx, double particle_volume);
  * V_p <= 0 or not finite: PICGPU_E_ARG. */
 int picgpu_set_particle_volume(picgpu_ctx *ctx, double particle_volume);
 
+/* ---- a locate that allocates nothing ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the walk of picgpu_locate over n particles the caller keeps in
+ * device memory, in the library's own layouts:
+ *   d_pos    double [n][3] positions (read)
+ *   d_cell   int32 [n] packed cell i + IM*(j + JM*k), -1 none: the start cell
+ *            in (-1: (0,0,0), walkingsearch.c:611-616), the located cell out
+ *   d_weight [n][3] weights out, double in PICGPU_FP64 contexts, float in PICGPU_FP32
+ *   d_status int32 [n] statuses in and out; ACTIVE_AND_LOCATED particles are
+ *            skipped, the others come out ACTIVE_AND_LOCATED or LOST
+ * The walk's counters and iteration histogram live in a fixed 40 KiB device
+ * array of the library (one per device, held until the process ends); the
+ * call makes no hipMalloc, hipFree or allocating library call.  Calls that
+ * overlap on one device race on that array.  The grid's face-plane records
+ * must already exist (any picgpu_locate or picgpu_step on the grid builds
+ * them): else PICGPU_E_STATE.  n above PICGPU_NOALLOC_MAX_PARTICLES:
+ * PICGPU_E_ARG.  st NULL: the call returns once the kernels are queued on the
+ * context's stream (picgpu_synchronize waits); st given: it waits and fills
+ * located, lost, walk_iterations, walk_max, fp64_fallbacks, n_particles = n
+ * and walk_hist, and reports a degenerate face as PICGPU_E_DEGENERATE. */
+#define PICGPU_NOALLOC_MAX_PARTICLES 2147483646LL
+int picgpu_locate_device_noalloc(picgpu_ctx *ctx, int64_t n, const double *d_pos, int32_t *d_cell, void *d_weight,
+                                 int32_t *d_status, picgpu_stats *st);
+
 /* ---- whole steps: reset, advect(+locate) [Euler|RK2], remove, interpolate, deposit ---- */
 int picgpu_step(picgpu_ctx *ctx, double dt, int nsteps, unsigned flags);
 
