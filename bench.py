@@ -33,18 +33,24 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(kernel: str, n: int, g: dict, rsize: int) -> float:
+def algorithmic_bytes(kernel: str, n: int, g: dict, rsize: int, deposit_share: float = 0.0) -> float:
     """Compulsory HBM bytes of one launch of `kernel` over n particles.
 
     The fused step moves the particle state once in and once out (S bytes
     each way: pos f64 24 + packed cell 4 + status 4, plus the FP64 velocity
     24 in the FP64 variant) and reads each node's coordinates (24 B) and
-    corner velocity (3 reals) once.  Grid-record caches, sort scratch and
-    re-reads are implementation choices and are not counted."""
+    corner velocity (3 reals) once.  Every launch of a call but the first
+    also deposits its particles' start state onto the faces (count and
+    momentum, 4 reals per face): deposit_share is the fraction of launches
+    that do.  Grid-record caches, sort scratch and re-reads are
+    implementation choices and are not counted."""
     Nn, Nc = g["nnode"], g["ncell"]
     S = 32 if rsize == 4 else 56
     if kernel.startswith("step_fused"):
-        return n * 2 * S + Nn * (24 + 3 * rsize)
+        return n * 2 * S + Nn * (24 + 3 * rsize) + deposit_share * g["nface"] * 4 * rsize
+    if kernel.startswith("step_finish"):
+        # position and cell read, weights and velocity written, faces written
+        return n * (24 + 4 + 6 * rsize) + Nn * (24 + 3 * rsize) + g["nface"] * 4 * rsize
     if kernel == "deposit_faces":
         # weights and velocity of each particle once; count + 3 momentum per face
         return n * 6 * rsize + g["nface"] * 4 * rsize
@@ -64,14 +70,15 @@ def algorithmic_bytes(kernel: str, n: int, g: dict, rsize: int) -> float:
 
 def design_bytes(kernel: str, n: int, g: dict, rsize: int) -> float:
     """Bytes the implementation itself moves per launch of the fused step:
-    particle fields (idx, status, cell, pos, velocity in; pos, velocity,
-    weights, cell, status, slot out), one face-plane record per cell (128 B
-    FP32, 304 B FP64) and the staged corner velocities."""
+    particle fields, one face-plane record per cell (128 B FP32, 304 B
+    FP64), the staged corner velocities and the faces."""
     if not kernel.startswith("step_fused"):
         return float("nan")
     rec = 128 if rsize == 4 else 304
-    return n * ((4 + 4 + 4 + 24 + 3 * rsize) + (24 + 3 * rsize + 3 * rsize + 4 + 4 + 4)) + \
-        g["ncell"] * rec + g["nnode"] * 3 * rsize
+    # in: index, cell, position; out: position, cell, slot; one plane record
+    # per cell, the staged corner velocities, the faces
+    return n * ((4 + 4 + 24) + (24 + 4 + 4)) + g["ncell"] * rec + g["nnode"] * 3 * rsize + \
+        g["nface"] * 4 * rsize
 
 
 def step_bytes(n: int, g: dict, rsize: int, faces: bool) -> float:
@@ -236,7 +243,8 @@ def run_variant(name: str, d: dict, steps: int, warmup: int, mlp: dict | None):
     g.AdvanceSimulation(d["dt"], res["breakdown_steps"], **flags)
     g.synchronize()
     kernels = {}
-    for k in ("step_fused_fp64", "step_fused_fp32", "step_list_fp64", "step_list_fp32", "key_histogram",
+    for k in ("step_fused_fp64", "step_fused_fp32", "step_finish_fp64", "step_finish_fp32", "face_seams",
+              "step_list_fp64", "step_list_fp32", "key_histogram",
               "key_scatter", "key_reindex", "step_deferred", "cell_planes", "bucket_pid_order", "iota", "gather_swarm", "locate_fp64", "locate_fp32",
               "interp", "center_to_corner", "rk2_predict", "rk2_correct", "keep_flags", "compact", "cell_histogram",
               "sort_scatter", "scan_reduce", "scan_tile", "scatter_P", "deposit_faces", "reset_advect", "compact_tail",
@@ -382,7 +390,9 @@ def roofline_for(res: dict, d: dict, g: dict) -> dict:
     kern = res["dominant_kernel_timed"]  # the step kernel, timed over the timed steps
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     n_avg = res["particle_steps"] / max(1, res["steps"])
-    b = algorithmic_bytes(dom, int(n_avg), g, rsize)
+    faces = "faces" in d["deposit"] and "ucont" not in d  # per-step field preparation: a pass of its own
+    share = (res["steps"] - 1) / res["steps"] if faces and res["steps"] > 0 else 0.0
+    b = algorithmic_bytes(dom, int(n_avg), g, rsize, share)
     achieved = b / (kern[dom]["avg_ms"] * 1e-3) / 1e9
     return {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,

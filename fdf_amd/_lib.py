@@ -44,6 +44,7 @@ STEP_DEPOSIT_CELLS = 0x2
 STEP_DEPOSIT_FACES = 0x4
 STEP_FIELDS = 0x8
 STEP_STAGED = 0x100
+STEP_FACES_OF_START = 0x20
 
 
 class PicgpuOptions(C.Structure):

@@ -50,6 +50,30 @@ __device__ __forceinline__ int32_t cell_key(const TileMap &t, int i, int j, int 
   return (((tk * t.NTJ + tj) * t.NTI + ti) * TS) + ((lk * TY + lj) * TX + li);
 }
 
+// Face deposition output of a tiled kernel.  A workgroup writes the faces
+// whose two cells are both in its tile, and the faces at the domain's
+// boundary; for a face between two tiles it writes its half (the lower
+// cell's sum to hi, the upper cell's to lo), and k_face_seams adds the
+// halves.  Seam faces of direction d lie on the planes m * T_d, m = 1 ..
+// NT_d - 1 (coordinate < the cell count); seam entry (m, a, b) holds 4 reals:
+//   i: ((m-1) KM + k) JM + j,  j: ((m-1) KM + k) IM + i,  k: ((m-1) JM + j) IM + i.
+template <typename R>
+struct FaceOut {
+  R *face[3];     // i-faces [KM][JM][IM+1], j [KM][JM+1][IM], k [KM+1][JM][IM], 4 reals each
+  R *lo[3], *hi[3];
+};
+
+__device__ __forceinline__ int64_t seam_entry(int d, int m, int i, int j, int k, int IM, int JM, int KM) {
+  if (d == 0) return ((int64_t)(m - 1) * KM + k) * JM + j;
+  if (d == 1) return ((int64_t)(m - 1) * KM + k) * IM + i;
+  return ((int64_t)(m - 1) * JM + j) * IM + i;
+}
+__device__ __forceinline__ int64_t face_index(int d, int i, int j, int k, int IM, int JM) {
+  if (d == 0) return ((int64_t)k * JM + j) * (IM + 1) + i;
+  if (d == 1) return ((int64_t)k * (JM + 1) + j) * IM + i;
+  return ((int64_t)k * JM + j) * IM + i;
+}
+
 template <typename R>
 struct StepArgs {
   GridView g;
@@ -74,6 +98,15 @@ struct StepArgs {
   unsigned long long *walk_hist;  // null, or [kCounterStripes][kWalkBins] walk-iteration histogram
   int all_located;  // 1 after the first step of a call: every particle of the index is
                     // ACTIVE_AND_LOCATED, statuses are neither read nor written
+  // k_step_tiled<..., WU = false>: the start velocity u0 is read from
+  // vel (u0_recompute 0: the first step of a call) or recomputed from the
+  // particle's position and cell (1: every later step; vel and w are then not
+  // kept between the steps of a call); with deposit 1 the workgroup also
+  // deposits its particles' start state onto the faces, which is the
+  // previous step's face deposition (FaceOut)
+  int u0_recompute;
+  int deposit;
+  FaceOut<R> fo;
 };
 
 constexpr int kWalkBins = 64;
@@ -329,14 +362,15 @@ struct PState {
   R u0[3];
 };
 
+// read_vel false: u0 is left for the caller to recompute
 template <typename R>
-__device__ __forceinline__ void load_state(const StepArgs<R> &a, int64_t p, PState<R> &s) {
+__device__ __forceinline__ void load_state(const StepArgs<R> &a, int64_t p, PState<R> &s, bool read_vel = true) {
   s.st = a.all_located ? ST_LOCATED : a.status[p];
   s.c0 = a.cell[p];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     s.x0[c] = a.pos[3 * p + c];
-    s.u0[c] = a.vel[3 * p + c];
+    s.u0[c] = read_vel ? a.vel[3 * p + c] : (R)0;
   }
 }
 
@@ -460,6 +494,13 @@ template <typename R>
 struct OutCap {
   static constexpr int value = sizeof(R) == 4 ? 448 : 384;
 };
+// Start states (weights and velocity) of the first DepCap<R> particles of a
+// tile are kept in LDS for the tile's face sums; those of later particles
+// go to the w and vel arrays.
+template <typename R>
+struct DepCap {
+  static constexpr int value = sizeof(R) == 4 ? 512 : 352;
+};
 enum OutKind : int8_t {
   OUT_NONE = 0,   // deferred to k_step_deferred: nothing is written
   OUT_SLOT = 1,   // removed before the step: slot -1 only
@@ -468,18 +509,20 @@ enum OutKind : int8_t {
   OUT_OOB = 4,    // outside the bounding box: position, cell, weights, removed
   OUT_LOC = 5     // located: position, cell, weights, velocity, slot
 };
-template <typename R>
+// WU: weights and velocity are written (otherwise the next step recomputes them)
+template <typename R, bool WU>
 struct OutStage {
   static constexpr int N = OutCap<R>::value;
+  static constexpr int NW = WU ? N : 1;
   double pos[N][3];
-  R vel[N][3];
-  R w[N][3];
+  R vel[NW][3];
+  R w[NW][3];
   int32_t cell[N];
   int32_t slot[N];
   int8_t kind[N];
 };
 
-template <typename R>
+template <typename R, bool WU>
 __device__ __forceinline__ void emit_global(const StepArgs<R> &a, int64_t p, int kind, const double x[3],
                                             int32_t cell, const R w[3], const R u[3], int32_t slot) {
   if (kind == OUT_NONE) return;
@@ -492,12 +535,12 @@ __device__ __forceinline__ void emit_global(const StepArgs<R> &a, int64_t p, int
     a.cell[p] = cell;
     if (!a.all_located) a.status[p] = kind == OUT_LOC ? ST_LOCATED : ST_REMOVED;
   }
-  if (kind >= OUT_OOB) {
+  if (WU && kind >= OUT_OOB) {
     a.w[3 * p] = w[0];
     a.w[3 * p + 1] = w[1];
     a.w[3 * p + 2] = w[2];
   }
-  if (kind == OUT_LOC) {
+  if (WU && kind == OUT_LOC) {
     a.vel[3 * p] = u[0];
     a.vel[3 * p + 1] = u[1];
     a.vel[3 * p + 2] = u[2];
@@ -505,53 +548,342 @@ __device__ __forceinline__ void emit_global(const StepArgs<R> &a, int64_t p, int
   a.slot[p] = slot;
 }
 
-// One workgroup per tile: stage the tile's grid data in LDS, then step the
-// tile's particles idx[kstart[t*TS] .. kstart[(t+1)*TS]), lane l taking
-// particles l, l + kBlock, ...  Each lane runs one loop over all its walk
-// iterations (predictor walk, corrector walk, next particle), so that the
-// lanes of a wavefront evaluate cells together whatever the walk lengths of
-// their particles; the next particle's state is requested one particle ahead.
-// Particles that stay in the tile count into an LDS histogram of the tile's
-// TS keys, reserved in the global histogram once per key at the end.
-template <typename R, bool EXACT, bool RK2>
+// Start state of a located particle at x0 in its cell (the record index lc,
+// cell (ci, cj, ck)): the weights of x0 (ParticleSwarm.c:1010-1047) and the
+// trilinear velocity (interpolation.c:957-1042), with the arithmetic the
+// step that located it used, so the same bits.  Where the FP32 filter does
+// not certify a face, the step that located the particle ran it through the
+// FP64 routine (k_step_deferred or a list kernel), which left the weights
+// and velocity in w and vel: they are read from there.
+template <typename R, bool EXACT, class Src>
+__device__ __forceinline__ void start_state(const StepArgs<R> &a, const Src &src, int64_t p, int32_t lc, int ci,
+                                            int cj, int ck, const double x0[3], R w0[3], R u0[3]) {
+  R d[6];
+  bool certain = true;
+  if constexpr (EXACT)
+    eval_planes_exact(src.planes[lc], x0, d);
+  else
+    certain = planes_fp32_distances(src.planes[lc], x0, d) == 0u;
+  if (certain) {
+    weights_from_distances<R>(d, a.clamp, w0);
+    interp_at<R>(src, ci, cj, ck, w0, u0);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      w0[c] = a.w[3 * p + c];
+      u0[c] = a.vel[3 * p + c];
+    }
+  }
+}
+
+// The tile's face deposition (no reference counterpart; the oracle's
+// ref_deposit_faces).  Thread l sums the start states of cell l's particles,
+// index positions kstart[t TS + l] .. kstart[t TS + l + 1] in PID order (from
+// the LDS stage sd below DepCap, from w and vel beyond), once with weight
+// 1 - a (the cell's lower faces) and once with weight a (its upper faces),
+// per direction; a face is its lower cell's sum plus its upper cell's.  The
+// lower neighbour's sums come through LDS (xch) when it is in the tile; a
+// face between two tiles gets its halves in the seam arrays (FaceOut).
+// Every thread of the workgroup calls it, after a barrier that follows the
+// last write of a start state.
+template <typename R, int CAPD>
+__device__ __forceinline__ void tile_face_sums(const StepArgs<R> &a, int t, int32_t s,
+                                               const int32_t *__restrict__ idx, const int32_t *__restrict__ kstart,
+                                               R (*sd)[6], R (*xch)[TS]) {
+  const int l = threadIdx.x;
+  const int li = l % TX, lj = (l / TX) % TY, lk = l / (TX * TY);
+  const int ti = t % a.tm.NTI, tj = (t / a.tm.NTI) % a.tm.NTJ, tk = t / (a.tm.NTI * a.tm.NTJ);
+  const int IM = a.g.IM, JM = a.g.JM, KM = a.g.KM;
+  const int i = ti * TX + li, j = tj * TY + lj, k = tk * TZ + lk;
+  const bool own = i < IM && j < JM && k < KM;
+  FaceAcc<R> hl[3][4], hh[3][4];
+  if (own) {
+    const int64_t key = (int64_t)t * TS + l;
+    const int32_t b0 = kstart[key], b1 = kstart[key + 1];
+    for (int32_t q = b0; q < b1; ++q) {
+      R w[3], u[3];
+      const int32_t lq = q - s;
+      if (lq < CAPD) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          w[c] = sd[lq][c];
+          u[c] = sd[lq][3 + c];
+        }
+      } else {
+        const int64_t p = idx[q];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          w[c] = a.w[3 * p + c];
+          u[c] = a.vel[3 * p + c];
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const R wa = w[d], wb = (R)1 - w[d];
+        hh[d][0].add(wa);
+        hh[d][1].add(wa * u[0]);
+        hh[d][2].add(wa * u[1]);
+        hh[d][3].add(wa * u[2]);
+        hl[d][0].add(wb);
+        hl[d][1].add(wb * u[0]);
+        hl[d][2].add(wb * u[1]);
+        hl[d][3].add(wb * u[2]);
+      }
+    }
+  }
+  const int g3[3] = {i, j, k}, l3[3] = {li, lj, lk}, dims[3] = {IM, JM, KM};
+  constexpr int T3[3] = {TX, TY, TZ}, STR[3] = {1, TX, TX * TY};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    __syncthreads();  // xch is free (and, the first time, the caller's LDS use is over)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) xch[v][l] = hh[d][v].value();
+    __syncthreads();
+    if (!own) continue;
+    R *face = a.fo.face[d];
+    const int64_t fl = face_index(d, i, j, k, IM, JM);
+    if (l3[d] > 0) {  // lower neighbour in the tile
+#pragma unroll
+      for (int v = 0; v < 4; ++v) face[4 * fl + v] = xch[v][l - STR[d]] + hl[d][v].value();
+    } else if (g3[d] == 0) {  // the domain's first face: no lower cell
+#pragma unroll
+      for (int v = 0; v < 4; ++v) face[4 * fl + v] = (R)0 + hl[d][v].value();
+    } else {
+      const int64_t se = seam_entry(d, g3[d] / T3[d], i, j, k, IM, JM, KM);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) a.fo.lo[d][4 * se + v] = hl[d][v].value();
+    }
+    if (g3[d] + 1 == dims[d]) {  // the domain's last face: no upper cell
+      const int64_t fu = face_index(d, i + (d == 0), j + (d == 1), k + (d == 2), IM, JM);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) face[4 * fu + v] = hh[d][v].value() + (R)0;
+    } else if (l3[d] == T3[d] - 1) {  // upper neighbour in the next tile
+      const int64_t se = seam_entry(d, (g3[d] + 1) / T3[d], i, j, k, IM, JM, KM);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) a.fo.hi[d][4 * se + v] = hh[d][v].value();
+    }
+  }
+}
+
+// Particles the fast pass hands to the general loop, per tile (index
+// positions in LDS); more go to k_step_deferred.
+constexpr int kSlowCap = 256;
+
+// One workgroup per tile: stage the tile's corner velocities in LDS, then
+// step the tile's particles idx[kstart[t*TS] .. kstart[(t+1)*TS]) in two
+// passes.
+//
+// Fast pass (lane l takes index positions l, l + kBlock, ...): a particle
+// whose walks each end in the cell they start from -- the predictor and the
+// corrector both accept the particle's own cell at the first evaluation,
+// every face certified in FP32 mode -- is stepped in straight-line code; one
+// plane record serves the start state, the predictor and the corrector.
+// With WU = false the step ends there: the next step recomputes the
+// weights and velocity at the new position.  Every other particle (a walk
+// that leaves the cell, an uncertified face, a degenerate cell, a particle
+// that is not located at the start of the step) is queued for the general
+// loop.
+//
+// General loop (lanes take the queued particles): all walk iterations of a
+// lane, across its particles, are one loop (predictor walk, corrector walk,
+// next particle), so that the lanes of a wavefront evaluate cells together
+// whatever the walk lengths of their particles; the next particle's state is
+// requested one particle ahead.  Particles that stay in the tile count into
+// an LDS histogram of the tile's TS keys, reserved in the global histogram
+// once per key at the end; particles that leave it take a global slot.
+//
+// WU = false (every step of a call except with the per-step field
+// preparation): weights and velocities are not written; each particle's
+// start velocity is read from vel (a.u0_recompute 0: the first step of a
+// call) or recomputed from its position and cell (1), which gives the bits
+// the previous step computed.  With a.deposit the start states of the
+// tile's particles -- the state the previous step left -- are also
+// deposited onto the faces (tile_face_sums), so the previous step's
+// deposition costs no second pass over the particles.
+// WU = true: weights and velocities are written and the start velocity is
+// read (the field may change between steps); deposition is a kernel of its own.
+template <typename R, bool EXACT, bool RK2, bool WU>
 __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int32_t *__restrict__ idx,
                                                        const int32_t *__restrict__ kstart,
                                                        int32_t *__restrict__ kbase) {
   static_assert(TS == kBlock, "one thread per key of the tile");
   constexpr int CAP = OutCap<R>::value;
+  constexpr int CAPD = WU ? 1 : DepCap<R>::value;
   __shared__ R sk[BN * 3];
   __shared__ int32_t scount[TS];
-  __shared__ OutStage<R> so;
+  __shared__ union {
+    OutStage<R, WU> so;
+    R xch[4][TS];
+  } su;
+  __shared__ R sd[CAPD][6];
   __shared__ int32_t shist[kWalkBins];
+  __shared__ int32_t squeue[kSlowCap];
+  __shared__ int32_t sqn;
+  auto &so = su.so;
   const int t = blockIdx.x;
   const int32_t s = kstart[(int64_t)t * TS], e = kstart[(int64_t)(t + 1) * TS];
-  if (s == e) return;  // empty tile (uniform over the block)
+  const bool dep = !WU && a.deposit;
+  const bool recompute = !WU && a.u0_recompute;
+  if (s == e && !dep) return;  // empty tile (uniform over the block)
   const int ti = t % a.tm.NTI, tj = (t / a.tm.NTI) % a.tm.NTJ, tk = t / (a.tm.NTI * a.tm.NTJ);
   PlaneSrc<R, true, !EXACT> src{a.g, a.corner, (const typename PlanesOf<R>::type *)a.planes, (lds_ptr<R>)sk,
                                ti * TX - 1, tj * TY - 1, tk * TZ - 1};
   const int32_t B = (int32_t)blockDim.x;
-  // first particle, and the number of the second, requested before staging
-  int32_t q = s + (int32_t)threadIdx.x;
-  int32_t p = q < e ? idx[q] : -1;
-  int32_t pn = q + B < e ? idx[q + B] : -1;
-  PState<R> ps;
-  if (p >= 0) load_state(a, p, ps);
-  const int64_t sy = a.g.IM + 1, sz = (int64_t)(a.g.JM + 1) * (a.g.IM + 1);
-  for (int u = threadIdx.x; u < BN; u += blockDim.x) {
-    const int bx = u % BX, by = (u / BX) % BY, bz = u / (BX * BY);
-    const int ni = src.i0 + bx, nj = src.j0 + by, nk = src.k0 + bz;
-    if (ni >= 0 && ni <= a.g.IM && nj >= 0 && nj <= a.g.JM && nk >= 0 && nk <= a.g.KM) {
-      const int64_t node = (int64_t)nk * sz + (int64_t)nj * sy + ni;
+  if (s < e) {
+    const int64_t sy = a.g.IM + 1, sz = (int64_t)(a.g.JM + 1) * (a.g.IM + 1);
+    for (int u = threadIdx.x; u < BN; u += blockDim.x) {
+      const int bx = u % BX, by = (u / BX) % BY, bz = u / (BX * BY);
+      const int ni = src.i0 + bx, nj = src.j0 + by, nk = src.k0 + bz;
+      if (ni >= 0 && ni <= a.g.IM && nj >= 0 && nj <= a.g.JM && nk >= 0 && nk <= a.g.KM) {
+        const int64_t node = (int64_t)nk * sz + (int64_t)nj * sy + ni;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) sk[3 * u + c] = a.corner[3 * node + c];
+        for (int c = 0; c < 3; ++c) sk[3 * u + c] = a.corner[3 * node + c];
+      }
     }
   }
   scount[threadIdx.x] = 0;
   if (threadIdx.x < kWalkBins) shist[threadIdx.x] = 0;
+  if (threadIdx.x == 0) sqn = 0;
   __syncthreads();
 
   StepTally tally;
   const double h = 0.5 * a.dt;
+  // a finished particle's result: staged in LDS (index order) or written now
+  auto finish = [&](int32_t q, int64_t p, int kind, const double x[3], int32_t cell, const R w[3], const R u[3],
+                    int32_t slot) {
+    const int32_t lq = q - s;
+    if (lq < CAP) {
+      so.kind[lq] = (int8_t)kind;
+      so.pos[lq][0] = x[0];
+      so.pos[lq][1] = x[1];
+      so.pos[lq][2] = x[2];
+      so.cell[lq] = cell;
+      so.slot[lq] = slot;
+      if constexpr (WU) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          so.w[lq][c] = w[c];
+          so.vel[lq][c] = u[c];
+        }
+      }
+    } else {
+      emit_global<R, WU>(a, p, kind, x, cell, w, u, slot);
+    }
+  };
+  // the start state's deposition entry of index position q
+  auto stage_dep = [&](int32_t q, int64_t p, const R w0[3], const R u0[3]) {
+    const int32_t lq = q - s;
+    if (lq < CAPD) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        sd[lq][c] = w0[c];
+        sd[lq][3 + c] = u0[c];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        a.w[3 * p + c] = w0[c];
+        a.vel[3 * p + c] = u0[c];
+      }
+    }
+  };
+
+  // ---- fast pass ----
+  for (int32_t q = s + (int32_t)threadIdx.x; q < e; q += B) {
+    const int32_t p = idx[q];
+    PState<R> ps;
+    load_state(a, p, ps, !recompute);
+    bool fast = ps.st == ST_LOCATED && ps.c0 >= 0;
+    if (fast) {
+      const int32_t lc = ps.c0;
+      int ci, cj, ck;
+      unpack_cell(lc, cell_idx(a.g), ci, cj, ck);
+      // FP32: the record, read once into registers, serves every evaluation;
+      // FP64: its 304 bytes are read where they are used
+      CellPlanes32 rec32;
+      if constexpr (!EXACT) rec32 = static_cast<const CellPlanes32 *>(a.planes)[lc];
+      auto dist = [&](const double x[3], R d[6]) -> bool {  // certified distances
+        if constexpr (EXACT)
+          return eval_planes_exact(src.planes[lc], x, d);
+        else
+          return planes_fp32_distances(rec32, x, d) == 0u;
+      };
+      if (recompute) {
+        R d0[6], w0[3];
+        if (dist(ps.x0, d0)) {
+          weights_from_distances<R>(d0, a.clamp, w0);
+          interp_at<R>(src, ci, cj, ck, w0, ps.u0);
+        } else {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            w0[c] = a.w[3 * p + c];
+            ps.u0[c] = a.vel[3 * p + c];
+          }
+        }
+        if (dep) stage_dep(q, p, w0, ps.u0);
+      }
+      double x1[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) x1[c] = ps.x0[c] + (double)ps.u0[c] * a.dt;
+      R d[6];
+      fast = dist(x1, d) && point_accepted<R>(d);
+      if (RK2 && fast) {
+        R ws[3], us[3];
+        weights_from_distances<R>(d, a.clamp, ws);
+        interp_at<R>(src, ci, cj, ck, ws, us);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x1[c] = ps.x0[c] + ((double)ps.u0[c] + (double)us[c]) * h;
+        fast = dist(x1, d) && point_accepted<R>(d);
+      }
+      if (fast) {
+        R w1[3] = {0, 0, 0}, u1[3] = {0, 0, 0};
+        if constexpr (WU) {
+          weights_from_distances<R>(d, a.clamp, w1);
+          interp_at<R>(src, ci, cj, ck, w1, u1);
+        }
+        const int it = RK2 ? 2 : 1;
+        tally.iters += it;
+        if (a.walk_hist) atomicAdd(&shist[walk_bin(it)], 1);
+        const bool in = x1[0] >= a.bx0 && x1[0] <= a.bx1 && x1[1] >= a.by0 && x1[1] <= a.by1 && x1[2] >= a.bz0 &&
+                        x1[2] <= a.bz1;
+        int32_t slot = -1;
+        if (in) {
+          tally.located++;
+          slot = atomicAdd(&scount[cell_key(a.tm, ci, cj, ck) - t * TS], 1) | kLocalSlot;
+        } else {
+          tally.oob++;
+        }
+        finish(q, p, in ? OUT_LOC : OUT_OOB, x1, lc, w1, u1, slot);
+      }
+    }
+    if (!fast) {
+      const int r = atomicAdd(&sqn, 1);
+      if (r < kSlowCap) {
+        squeue[r] = q;
+      } else {
+        // more than the queue holds: stepped by k_step_deferred, which
+        // reads the start velocity from vel
+        a.defer_list[atomicAdd(a.defer_count, 1)] = p;
+        if (recompute) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) a.vel[3 * p + c] = ps.u0[c];
+        }
+        finish(q, p, OUT_NONE, ps.x0, -1, ps.u0, ps.u0, -1);
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- general loop over the queued particles ----
+  const int32_t nq = sqn < kSlowCap ? sqn : kSlowCap;
+  int32_t r = (int32_t)threadIdx.x;  // queue entry of the current particle
+  auto qpos = [&](int32_t rr) -> int32_t { return rr < nq ? squeue[rr] : -1; };
+  int32_t q = qpos(r);
+  int32_t p = q >= 0 ? idx[q] : -1;
+  int32_t qn = qpos(r + B);
+  int32_t pn = qn >= 0 ? idx[qn] : -1;
+  PState<R> ps;
+  if (p >= 0) load_state(a, p, ps, !recompute);
   PState<R> pnext;  // state of particle pn, requested one particle ahead
   int32_t pnn = -1;
   // per-particle walk state; phase 0: no particle, 1: predictor walk (RK2),
@@ -566,13 +898,26 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
 
   auto begin = [&]() {
     // request the next particle's state, then set up the walk of p
-    if (pn >= 0) load_state(a, pn, pnext);
-    pnn = q + 2 * B < e ? idx[q + 2 * B] : -1;
+    if (pn >= 0) load_state(a, pn, pnext, !recompute);
+    const int32_t qnn = qpos(r + 2 * B);
+    pnn = qnn >= 0 ? idx[qnn] : -1;
     ci = cj = ck = 0;
     if (ps.c0 >= 0) unpack_cell(ps.c0, cell_idx(a.g), ci, cj, ck);
     lc = ps.c0 >= 0 ? ps.c0 : 0;
     steps = stot = 0;
     cyc = CycleGuard{};
+    if (recompute) {
+      // the start velocity, as the fast pass computed it (every particle of
+      // the index is located)
+      const int32_t lq = q - s;
+      if (dep && lq < CAPD) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ps.u0[c] = sd[lq][3 + c];
+      } else {
+        R w0[3];
+        start_state<R, EXACT>(a, src, p, lc, ci, cj, ck, ps.x0, w0, ps.u0);
+      }
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) xw[c] = ps.x0[c] + (double)ps.u0[c] * a.dt;
     if (ps.st == ST_REMOVED)
@@ -592,16 +937,16 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
   if (p >= 0) begin();
 
   while (ph != 0) {
-    int r;
+    int rw;
     R ui[3] = {0, 0, 0};
     if (ph >= 3) {
-      r = W_LOST;
+      rw = W_LOST;
     } else {
-      r = walk_iter_lin<R, EXACT>(src, xw, ci, cj, ck, lc, steps, cyc, a.max_traversal, a.thresh_scale, a.clamp, wv,
-                              tally.fb);
-      if (r == W_CONTINUE) continue;
-      if (r == W_LOCATED) interp_at<R>(src, ci, cj, ck, wv, ui);
-      if (RK2 && ph == 1 && r == W_LOCATED) {
+      rw = walk_iter_lin<R, EXACT>(src, xw, ci, cj, ck, lc, steps, cyc, a.max_traversal, a.thresh_scale, a.clamp,
+                                   wv, tally.fb);
+      if (rw == W_CONTINUE) continue;
+      if (rw == W_LOCATED) interp_at<R>(src, ci, cj, ck, wv, ui);
+      if (RK2 && ph == 1 && rw == W_LOCATED) {
         // corrector position; the final walk starts from the predictor's cell
 #pragma unroll
         for (int c = 0; c < 3; ++c) xw[c] = ps.x0[c] + ((double)ps.u0[c] + (double)ui[c]) * h;
@@ -612,12 +957,17 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
         continue;
       }
     }
-    // particle p is done: outcome r at position xw (x0 when LOST at start)
+    // particle p is done: outcome rw at position xw (x0 when LOST at start)
     int kind;
     int32_t cell = -1, slot = -1;
-    if (r == W_DEFER) {
-      // stepped again from its start state by k_step_deferred
+    if (rw == W_DEFER) {
+      // stepped again from its start state by k_step_deferred, which reads
+      // the start velocity from vel
       a.defer_list[atomicAdd(a.defer_count, 1)] = p;
+      if (recompute) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a.vel[3 * p + c] = ps.u0[c];
+      }
       kind = OUT_NONE;
     } else if (ph == 4) {
       kind = OUT_SLOT;
@@ -628,10 +978,10 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
       }
       tally.iters += stot + steps;
       if (a.walk_hist) atomicAdd(&shist[walk_bin(stot + steps)], 1);
-      if (r == W_DEGENERATE) {
+      if (rw == W_DEGENERATE) {
         tally.err = 1;
         kind = OUT_DEGEN;
-      } else if (r == W_LOST) {
+      } else if (rw == W_LOST) {
         tally.lost++;
         kind = OUT_LOST;
       } else {
@@ -655,24 +1005,11 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
         }
       }
     }
-    const int32_t lq = q - s;
-    if (lq < CAP) {
-      so.kind[lq] = (int8_t)kind;
-      so.pos[lq][0] = xw[0];
-      so.pos[lq][1] = xw[1];
-      so.pos[lq][2] = xw[2];
-      so.cell[lq] = cell;
-      so.slot[lq] = slot;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        so.w[lq][c] = wv[c];
-        so.vel[lq][c] = ui[c];
-      }
-    } else {
-      emit_global(a, p, kind, xw, cell, wv, ui, slot);
-    }
+    finish(q, p, kind, xw, cell, wv, ui, slot);
     // next particle
-    q += B;
+    r += B;
+    q = qn;
+    qn = qpos(r + B);
     p = pn;
     pn = pnn;
     ph = 0;
@@ -690,9 +1027,15 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
     const int kind = so.kind[lq];
     if (kind == OUT_NONE) continue;
     const double x[3] = {so.pos[lq][0], so.pos[lq][1], so.pos[lq][2]};
-    const R w[3] = {so.w[lq][0], so.w[lq][1], so.w[lq][2]};
-    const R u[3] = {so.vel[lq][0], so.vel[lq][1], so.vel[lq][2]};
-    emit_global(a, (int64_t)idx[s + lq], kind, x, so.cell[lq], w, u, so.slot[lq]);
+    R w[3] = {0, 0, 0}, u[3] = {0, 0, 0};
+    if constexpr (WU) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        w[c] = so.w[lq][c];
+        u[c] = so.vel[lq][c];
+      }
+    }
+    emit_global<R, WU>(a, (int64_t)idx[s + lq], kind, x, so.cell[lq], w, u, so.slot[lq]);
   }
   const int32_t cnt = scount[threadIdx.x];
   if (cnt > 0) {
@@ -702,6 +1045,104 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
   if (a.walk_hist && threadIdx.x < kWalkBins && shist[threadIdx.x] > 0)
     atomicAdd(&a.walk_hist[(blockIdx.x & (kCounterStripes - 1)) * kWalkBins + threadIdx.x],
               (unsigned long long)shist[threadIdx.x]);
+  if constexpr (!WU) {
+    if (dep) tile_face_sums<R, CAPD>(a, t, s, idx, kstart, sd, su.xch);
+  }
+}
+
+// The end of a step call (and the stage after any change of the swarm that
+// the steps do not make): every particle's weights and velocity at its
+// position and cell, written to w and vel, and, with a.deposit, the face
+// deposition of that state.  One workgroup per tile, as k_step_tiled.
+template <typename R, bool EXACT>
+__global__ __launch_bounds__(kBlock) void k_step_finish(StepArgs<R> a, const int32_t *__restrict__ idx,
+                                                        const int32_t *__restrict__ kstart) {
+  constexpr int CAPD = DepCap<R>::value;
+  __shared__ R sk[BN * 3];
+  __shared__ R xch[4][TS];
+  __shared__ R sd[CAPD][6];
+  const int t = blockIdx.x;
+  const int32_t s = kstart[(int64_t)t * TS], e = kstart[(int64_t)(t + 1) * TS];
+  if (s == e && !a.deposit) return;
+  const int ti = t % a.tm.NTI, tj = (t / a.tm.NTI) % a.tm.NTJ, tk = t / (a.tm.NTI * a.tm.NTJ);
+  PlaneSrc<R, true, !EXACT> src{a.g, a.corner, (const typename PlanesOf<R>::type *)a.planes, (lds_ptr<R>)sk,
+                               ti * TX - 1, tj * TY - 1, tk * TZ - 1};
+  if (s < e) {
+    const int64_t sy = a.g.IM + 1, sz = (int64_t)(a.g.JM + 1) * (a.g.IM + 1);
+    for (int u = threadIdx.x; u < BN; u += blockDim.x) {
+      const int bx = u % BX, by = (u / BX) % BY, bz = u / (BX * BY);
+      const int ni = src.i0 + bx, nj = src.j0 + by, nk = src.k0 + bz;
+      if (ni >= 0 && ni <= a.g.IM && nj >= 0 && nj <= a.g.JM && nk >= 0 && nk <= a.g.KM) {
+        const int64_t node = (int64_t)nk * sz + (int64_t)nj * sy + ni;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sk[3 * u + c] = a.corner[3 * node + c];
+      }
+    }
+    __syncthreads();
+    for (int32_t q = s + (int32_t)threadIdx.x; q < e; q += (int32_t)blockDim.x) {
+      const int64_t p = idx[q];
+      const int32_t lc = a.cell[p];
+      const double x0[3] = {a.pos[3 * p], a.pos[3 * p + 1], a.pos[3 * p + 2]};
+      int ci, cj, ck;
+      unpack_cell(lc, cell_idx(a.g), ci, cj, ck);
+      R w0[3], u0[3];
+      start_state<R, EXACT>(a, src, p, lc, ci, cj, ck, x0, w0, u0);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        a.w[3 * p + c] = w0[c];
+        a.vel[3 * p + c] = u0[c];
+      }
+      if (a.deposit && q - s < CAPD) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          sd[q - s][c] = w0[c];
+          sd[q - s][3 + c] = u0[c];
+        }
+      }
+    }
+  }
+  if (a.deposit) {
+    __syncthreads();  // every start state staged (sd) or written (w, vel)
+    tile_face_sums<R, CAPD>(a, t, s, idx, kstart, sd, xch);
+  }
+}
+
+// Faces between two tiles: the lower cell's sum (hi) plus the upper cell's
+// (lo).  Entries are numbered i-seams, then j-seams, then k-seams.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_face_seams(int IM, int JM, int KM, int64_t ni, int64_t nj, int64_t nk,
+                                                       FaceOut<R> fo) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int d, m, i = 0, j = 0, k = 0;
+  if (q < ni) {
+    d = 0;
+    j = (int)(q % JM);
+    const int64_t r = q / JM;
+    k = (int)(r % KM);
+    m = (int)(r / KM) + 1;
+    i = m * TX;
+  } else if ((q -= ni) < nj) {
+    d = 1;
+    i = (int)(q % IM);
+    const int64_t r = q / IM;
+    k = (int)(r % KM);
+    m = (int)(r / KM) + 1;
+    j = m * TY;
+  } else if ((q -= nj) < nk) {
+    d = 2;
+    i = (int)(q % IM);
+    const int64_t r = q / IM;
+    j = (int)(r % JM);
+    m = (int)(r / JM) + 1;
+    k = m * TZ;
+  } else {
+    return;
+  }
+  const int64_t f = face_index(d, i, j, k, IM, JM);
+  R *face = fo.face[d];
+  const R *lo = fo.lo[d] + 4 * q, *hi = fo.hi[d] + 4 * q;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) face[4 * f + v] = hi[v] + lo[v];
 }
 
 // Particles deferred by k_step_tiled, stepped from their start state with the
@@ -780,19 +1221,26 @@ __global__ __launch_bounds__(kBlock) void k_key_scatter(int64_t n, const int32_t
   idx[kstart[b] + s] = (int32_t)p;
 }
 
+// count: n, or *n_dev when n_dev is given (a count still on the device)
+__device__ __forceinline__ int64_t dev_count(int64_t n, const int32_t *n_dev) {
+  return n_dev ? (int64_t)*n_dev : n;
+}
+
 // After a step: idx_next[kstart_next[key(cell[p])] + slot[p]] = p for the
 // particles p = idx_cur[0..m) that survived it (slot >= 0; a slot in a
 // tile's own histogram is offset by the tile's base kbase[key]).  Each thread
 // takes kReindexPer entries, kBlock apart, and issues each round of its
 // dependent loads for all of them together.
 constexpr int kReindexPer = 4;
-__global__ __launch_bounds__(kBlock) void k_key_reindex(int64_t m, const int32_t *__restrict__ idx_cur,
+__global__ __launch_bounds__(kBlock) void k_key_reindex(int64_t m_max, const int32_t *__restrict__ m_dev,
+                                                        const int32_t *__restrict__ idx_cur,
                                                         const int32_t *__restrict__ cell,
                                                         const int32_t *__restrict__ slot,
                                                         const int32_t *__restrict__ kbase, CellIdx cx, TileMap tm,
                                                         const int32_t *__restrict__ kstart,
                                                         int32_t *__restrict__ idx_next) {
   constexpr int NP = kReindexPer;
+  const int64_t m = dev_count(m_max, m_dev);
   const int64_t q0 = (int64_t)blockIdx.x * (kBlock * NP) + threadIdx.x;
   int32_t p[NP], s[NP], c[NP], key[NP], base[NP];
 #pragma unroll
@@ -819,9 +1267,10 @@ __global__ __launch_bounds__(kBlock) void k_key_reindex(int64_t m, const int32_t
     if (s[u] >= 0) idx_next[base[u] + (s[u] & (kLocalSlot - 1))] = p[u];
 }
 
-__global__ __launch_bounds__(kBlock) void k_iota(int64_t n, int32_t *__restrict__ idx) {
+__global__ __launch_bounds__(kBlock) void k_iota(int64_t n, const int32_t *__restrict__ n_dev,
+                                                 int32_t *__restrict__ idx) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) idx[q] = (int32_t)q;
+  if (q < dev_count(n, n_dev)) idx[q] = (int32_t)q;
 }
 
 // Puts every bucket idx[kstart[key] .. kstart[key+1]) in ascending PID order,
@@ -1089,15 +1538,17 @@ __global__ __launch_bounds__(kBlock) void k_count_from_keys(int IM, int JM, int 
 }
 
 // Physical reorder: dst[q] = src[idx[q]] for q < m.  Between the steps of one
-// call (GATHER_STEP) weights and statuses are not moved: the next step
-// recomputes the weights and every particle is ACTIVE_AND_LOCATED.  At the
-// end of a call (GATHER_LAST) weights move and statuses are set to
-// ACTIVE_AND_LOCATED.  Each thread moves kGatherPer particles, kBlock apart:
-// all their loads are issued before the first store.
+// call (GATHER_STEP) statuses are not moved: every particle is
+// ACTIVE_AND_LOCATED.  At the end of a call (GATHER_LAST) statuses are set
+// to ACTIVE_AND_LOCATED.  Weights and velocities always move: between steps
+// they hold the start states that the next step cannot recompute in FP32.
+// Each thread moves kGatherPer particles, kBlock apart: all their loads are
+// issued before the first store.
 enum GatherMode : int { GATHER_ALL = 0, GATHER_STEP = 1, GATHER_LAST = 2 };
 constexpr int kGatherPer = 4;
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_gather_swarm(int64_t m, int mode, const int32_t *__restrict__ idx,
+__global__ __launch_bounds__(kBlock) void k_gather_swarm(int64_t m_max, const int32_t *__restrict__ m_dev, int mode,
+                                                         const int32_t *__restrict__ idx,
                                                          const double *__restrict__ pos, const R *__restrict__ vel,
                                                          const R *__restrict__ w, const int32_t *__restrict__ cell,
                                                          const int32_t *__restrict__ status,
@@ -1107,6 +1558,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_swarm(int64_t m, int mode, co
                                                          int32_t *__restrict__ status2, int64_t *__restrict__ pid2,
                                                          R *__restrict__ P2) {
   constexpr int NP = kGatherPer;
+  const int64_t m = dev_count(m_max, m_dev);
   const int64_t q0 = (int64_t)blockIdx.x * (kBlock * NP) + threadIdx.x;
   int64_t p[NP];
 #pragma unroll
@@ -1122,7 +1574,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_swarm(int64_t m, int mode, co
     for (int c = 0; c < 3; ++c) {
       x[u][c] = pos[3 * p[u] + c];
       v[u][c] = vel[3 * p[u] + c];
-      ww[u][c] = mode != GATHER_STEP ? w[3 * p[u] + c] : (R)0;
+      ww[u][c] = w[3 * p[u] + c];
     }
     ce[u] = cell[p[u]];
     st[u] = mode == GATHER_ALL ? status[p[u]] : ST_LOCATED;
@@ -1137,7 +1589,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_swarm(int64_t m, int mode, co
     for (int c = 0; c < 3; ++c) {
       pos2[3 * q + c] = x[u][c];
       vel2[3 * q + c] = v[u][c];
-      if (mode != GATHER_STEP) w2[3 * q + c] = ww[u][c];
+      w2[3 * q + c] = ww[u][c];
     }
     cell2[q] = ce[u];
     if (mode != GATHER_STEP) status2[q] = st[u];

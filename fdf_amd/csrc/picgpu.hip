@@ -272,6 +272,8 @@ int picgpu_destroy(picgpu_ctx *c) {
                     &c->kstart[0], &c->kstart[1], &c->kidx[0], &c->kidx[1], &c->kslot, &c->kbase, &c->planes,
                     &c->defer, &c->walk_hist};
   for (DevBuf *b : bufs) b->release();
+  c->step_counters.release();
+  c->seams.release();
   eulerian_release(c);
   SwarmBufs *sws[] = {&c->sw, &c->alt};
   for (SwarmBufs *s : sws) {
@@ -874,15 +876,16 @@ static int build_key_index(picgpu_ctx *c, const int32_t *guess) {
 }
 
 // Moves the live particles into key order (the order of kidx[kcur]); the
-// index becomes the identity.
-static int compact_by_key(picgpu_ctx *c, int64_t live, int mode) {
+// index becomes the identity.  live_dev: the live count when it is still on
+// the device (live is then an upper bound, and c->n is set by the caller).
+static int compact_by_key(picgpu_ctx *c, int64_t live, int mode, const int32_t *live_dev = nullptr) {
   PG_CHECK(ensure_swarm(c, c->alt, c->n));
   int32_t *idx = c->kidx[c->kcur].as<int32_t>();
   if (live > 0) {
     int r = dispatch_real(c, [&](auto *tag) -> int {
       using R = std::remove_pointer_t<decltype(tag)>;
-      PG_LAUNCH(c, "gather_swarm", nblocks(live, kBlock * kGatherPer), kBlock, k_gather_swarm<R>, live, mode, idx,
-                c->sw.pos.as<double>(),
+      PG_LAUNCH(c, "gather_swarm", nblocks(live, kBlock * kGatherPer), kBlock, k_gather_swarm<R>, live, live_dev, mode,
+                idx, c->sw.pos.as<double>(),
                 c->sw.vel.as<R>(), c->sw.w.as<R>(), c->sw.cell.as<int32_t>(), c->sw.status.as<int32_t>(),
                 c->sw.pid.as<int64_t>(), c->sw.P.as<R>(), c->alt.pos.as<double>(), c->alt.vel.as<R>(),
                 c->alt.w.as<R>(), c->alt.cell.as<int32_t>(), c->alt.status.as<int32_t>(), c->alt.pid.as<int64_t>(),
@@ -890,10 +893,10 @@ static int compact_by_key(picgpu_ctx *c, int64_t live, int mode) {
       return PICGPU_OK;
     });
     PG_CHECK(r);
-    PG_LAUNCH(c, "iota", nblocks(live), kBlock, k_iota, live, idx);
+    PG_LAUNCH(c, "iota", nblocks(live), kBlock, k_iota, live, live_dev, idx);
   }
   swap_swarm(c);
-  c->n = live;
+  if (!live_dev) c->n = live;
   c->key_ordered = true;
   return PICGPU_OK;
 }
@@ -913,16 +916,102 @@ int picgpu::settle_swarm(picgpu_ctx *c) {
 
 int64_t picgpu::live_particles(const picgpu_ctx *c) { return c->compact_pending ? c->n_live : c->n; }
 
+// Face arrays and the seam halves of the tiled face deposition.
+static int face_out(picgpu_ctx *c, void *fo_out) {
+  const TileMap tm = tile_map(c);
+  const int64_t ni = (int64_t)(tm.NTI - 1) * c->JM * c->KM, nj = (int64_t)(tm.NTJ - 1) * c->IM * c->KM,
+                nk = (int64_t)(tm.NTK - 1) * c->IM * c->JM;
+  for (int dir = 0; dir < 3; ++dir) PG_CHECK(c->face[dir].ensure(c->rsize * 4 * (size_t)nface(c, dir)));
+  PG_CHECK(c->seams.ensure(c->rsize * 4 * 2 * (size_t)(ni + nj + nk + 1)));
+  return dispatch_real(c, [&](auto *tag) -> int {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    FaceOut<R> &fo = *static_cast<FaceOut<R> *>(fo_out);
+    R *base = c->seams.as<R>();
+    const int64_t cnt[3] = {ni, nj, nk};
+    for (int d = 0; d < 3; ++d) {
+      fo.face[d] = c->face[d].as<R>();
+      fo.lo[d] = base;
+      fo.hi[d] = base + 4 * cnt[d];
+      base += 8 * cnt[d];
+    }
+    return PICGPU_OK;
+  });
+}
+
+static int launch_face_seams(picgpu_ctx *c, const void *fo_in) {
+  const TileMap tm = tile_map(c);
+  const int64_t ni = (int64_t)(tm.NTI - 1) * c->JM * c->KM, nj = (int64_t)(tm.NTJ - 1) * c->IM * c->KM,
+                nk = (int64_t)(tm.NTK - 1) * c->IM * c->JM;
+  if (ni + nj + nk == 0) return PICGPU_OK;
+  return dispatch_real(c, [&](auto *tag) -> int {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    const FaceOut<R> &fo = *static_cast<const FaceOut<R> *>(fo_in);
+    PG_LAUNCH(c, "face_seams", nblocks(ni + nj + nk), kBlock, k_face_seams<R>, c->IM, c->JM, c->KM, ni, nj, nk, fo);
+    return PICGPU_OK;
+  });
+}
+
+template <typename R>
+static void fill_step_args(picgpu_ctx *c, StepArgs<R> &a, double dt, unsigned long long *counters) {
+  a.g = grid_view(c);
+  a.pos = c->sw.pos.as<double>();
+  a.vel = c->sw.vel.as<R>();
+  a.w = c->sw.w.as<R>();
+  a.cell = c->sw.cell.as<int32_t>();
+  a.status = c->sw.status.as<int32_t>();
+  a.corner = c->corner.as<R>();
+  a.planes = c->planes.ptr;
+  a.dt = dt;
+  a.max_traversal = c->opt.max_traversal;
+  a.thresh_scale = c->opt.distance_threshold;
+  a.clamp = (R)c->opt.weight_clamp;
+  a.bx0 = c->bmin[0];
+  a.by0 = c->bmin[1];
+  a.bz0 = c->bmin[2];
+  a.bx1 = c->bmax[0];
+  a.by1 = c->bmax[1];
+  a.bz1 = c->bmax[2];
+  a.tm = tile_map(c);
+  a.count_next = c->kcount.as<int32_t>();
+  a.slot = c->kslot.as<int32_t>();
+  a.counters = counters;
+  a.walk_hist = nullptr;
+  a.all_located = 1;
+  a.u0_recompute = 0;
+  a.deposit = 0;
+  a.defer_list = nullptr;
+  a.defer_count = nullptr;
+  for (int d = 0; d < 3; ++d) a.fo.face[d] = a.fo.lo[d] = a.fo.hi[d] = nullptr;
+}
+
+// Steps of one picgpu_step call are queued without waiting for the device:
+// each step's statistics go to its own block of counters (kCounterStripes x
+// C_NUM words), and the live count that sizes the next step's launches stays
+// on the device (the total of the index, kstart[nkeys + 1]); the host reads
+// the counters once per kStepBatch steps.
+static constexpr int kStepBatch = 64;
+static constexpr size_t kStepCounterWords = (size_t)kCounterStripes * C_NUM;
+
 // The key index of the previous fused step stays valid until the swarm
 // changes (every entry point that changes cells or statuses clears keyed), so
 // a call that follows a step call starts from it.  Every particle of that
 // index is ACTIVE_AND_LOCATED; if the previous call ended between
 // compactions, the arrays still hold its removed particles (compact_pending)
 // and the compaction schedule (every kCompactEvery steps) carries on.
-// last: the call's last step, which compacts with weights and statuses if
-// the schedule falls on it.
-static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool last) {
+//
+// step: the step's number in the call; the first reads each particle's start
+// velocity from vel, later ones recompute it (unless the field is prepared
+// every step) and deposit the start state onto the faces, which is the
+// previous step's deposition (finish_step_call deposits the last one).
+// last: the call's last step, which compacts with weights and statuses if the
+// schedule falls on it.  counters: this step's counter block.  *compacted:
+// set when this step compacted the swarm.
+static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, int step, bool last,
+                         unsigned long long *counters, bool *compacted, int64_t *entries) {
   const bool rk2 = (flags & PICGPU_STEP_RK2) != 0;
+  const bool wu = (flags & PICGPU_STEP_FIELDS) != 0;
+  const bool faces = (flags & PICGPU_STEP_DEPOSIT_FACES) != 0;
+  *compacted = false;
   PG_CHECK(ensure_planes(c));
   const bool fresh = !c->keyed;  // index built here: statuses still to be read
   if (fresh) {
@@ -933,10 +1022,11 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool last) {
   }
   const TileMap tm = tile_map(c);
   const int64_t nt = ntiles(tm), nkeys = nt * TS;
-  const int64_t n = c->n, m = c->n_live;  // m: entries of the current index
+  const int64_t n = c->n;  // array length: an upper bound of the index entries
+  // entries of the current index, when the host knows them (a fresh index,
+  // or the first step of a batch)
+  *entries = fresh ? c->kend_unloc : c->n_live;
   const int cur = c->kcur, nxt = cur ^ 1;
-  c->stats.particle_steps += m;
-  PG_CHECK(counters_reset(c));
   PG_CHECK(c->kstart[nxt].ensure(sizeof(int32_t) * (size_t)(nkeys + 2)));
   PG_CHECK(c->kidx[nxt].ensure(sizeof(int32_t) * (size_t)(n > 0 ? n : 1)));
   PG_HIP(hipMemsetAsync(c->kcount.ptr, 0, sizeof(int32_t) * (size_t)(nkeys + 1), c->stream));
@@ -946,36 +1036,22 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool last) {
   unsigned long long *hist = hist_reset(c);
   const int32_t *idx = c->kidx[cur].as<int32_t>();
   const int32_t *kstart = c->kstart[cur].as<int32_t>();
+  const int32_t *m_dev = kstart + nkeys + 1;  // entries of the current index
   PG_CHECK(c->kbase.ensure(sizeof(int32_t) * (size_t)(nkeys > 0 ? nkeys : 1)));
   int32_t *kbase = c->kbase.as<int32_t>();
-  if (m > 0) {
+  const bool dep = !wu && faces && step > 0;
+  alignas(16) unsigned char fo_buf[sizeof(FaceOut<double>)];
+  if (dep) PG_CHECK(face_out(c, fo_buf));
+  if (n > 0) {
     int r = dispatch_real(c, [&](auto *tag) -> int {
       using R = std::remove_pointer_t<decltype(tag)>;
       StepArgs<R> a;
-      a.g = grid_view(c);
-      a.pos = c->sw.pos.as<double>();
-      a.vel = c->sw.vel.as<R>();
-      a.w = c->sw.w.as<R>();
-      a.cell = c->sw.cell.as<int32_t>();
-      a.status = c->sw.status.as<int32_t>();
-      a.corner = c->corner.as<R>();
-      a.planes = c->planes.ptr;
-      a.dt = dt;
-      a.max_traversal = c->opt.max_traversal;
-      a.thresh_scale = c->opt.distance_threshold;
-      a.clamp = (R)c->opt.weight_clamp;
-      a.bx0 = c->bmin[0];
-      a.by0 = c->bmin[1];
-      a.bz0 = c->bmin[2];
-      a.bx1 = c->bmax[0];
-      a.by1 = c->bmax[1];
-      a.bz1 = c->bmax[2];
-      a.tm = tm;
-      a.count_next = c->kcount.as<int32_t>();
-      a.slot = c->kslot.as<int32_t>();
-      a.counters = c->counters.as<unsigned long long>();
+      fill_step_args<R>(c, a, dt, counters);
       a.walk_hist = hist;
       a.all_located = fresh ? 0 : 1;
+      a.u0_recompute = !wu && step > 0;
+      a.deposit = dep ? 1 : 0;
+      if (dep) a.fo = *reinterpret_cast<const FaceOut<R> *>(fo_buf);
       a.defer_list = c->defer.as<int32_t>();
       a.defer_count = c->defer.as<int32_t>() + (n > 0 ? n : 1);
       constexpr bool EX = sizeof(R) == sizeof(double);
@@ -984,51 +1060,39 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool last) {
       const int64_t u0 = c->kbegin_unloc, u1 = c->kend_unloc;
       // deferred particles: at most the particles of the index; the kernel
       // reads their number on the device
-      const int64_t dblocks = std::min<int64_t>(1024, nblocks(m));
-      if (rk2) {
-        PG_LAUNCH(c, name, nt, kBlock, (k_step_tiled<R, EX, true>), a, idx, kstart, kbase);
-        if (u1 > u0) PG_LAUNCH(c, name_list, nblocks(u1 - u0), kBlock, (k_step_list<R, EX, true>), a, idx, u0, u1);
-        if (!EX) PG_LAUNCH(c, "step_deferred", dblocks, kBlock, (k_step_deferred<R, EX, true>), a);
-      } else {
-        PG_LAUNCH(c, name, nt, kBlock, (k_step_tiled<R, EX, false>), a, idx, kstart, kbase);
-        if (u1 > u0) PG_LAUNCH(c, name_list, nblocks(u1 - u0), kBlock, (k_step_list<R, EX, false>), a, idx, u0, u1);
-        if (!EX) PG_LAUNCH(c, "step_deferred", dblocks, kBlock, (k_step_deferred<R, EX, false>), a);
-      }
-      return PICGPU_OK;
+      const int64_t dblocks = std::min<int64_t>(1024, nblocks(n));
+      auto launch = [&](auto rk2_tag, auto wu_tag) -> int {
+        constexpr bool RK = decltype(rk2_tag)::value, WU = decltype(wu_tag)::value;
+        PG_LAUNCH(c, name, nt, kBlock, (k_step_tiled<R, EX, RK, WU>), a, idx, kstart, kbase);
+        if (fresh && u1 > u0)
+          PG_LAUNCH(c, name_list, nblocks(u1 - u0), kBlock, (k_step_list<R, EX, RK>), a, idx, u0, u1);
+        PG_LAUNCH(c, "step_deferred", dblocks, kBlock, (k_step_deferred<R, EX, RK>), a);
+        return PICGPU_OK;
+      };
+      using T = std::true_type;
+      using F = std::false_type;
+      if (rk2) return wu ? launch(T{}, T{}) : launch(T{}, F{});
+      return wu ? launch(F{}, T{}) : launch(F{}, F{});
     });
     PG_CHECK(r);
   }
+  if (dep) {
+    PG_CHECK(launch_face_seams(c, fo_buf));
+    c->have_faces = true;
+  }
   PG_CHECK(scan_exclusive_i32(c, c->kcount.as<int32_t>(), c->kstart[nxt].as<int32_t>(), nkeys + 1));
-  if (m > 0)
-    PG_LAUNCH(c, "key_reindex", nblocks(m, kBlock * kReindexPer), kBlock, k_key_reindex, m, idx,
+  if (n > 0)
+    PG_LAUNCH(c, "key_reindex", nblocks(n, kBlock * kReindexPer), kBlock, k_key_reindex, n, m_dev, idx,
               c->sw.cell.as<int32_t>(),
               c->kslot.as<int32_t>(), kbase, CellIdx{c->IM, c->JM, grid_view(c).rI, grid_view(c).rIJ}, tm,
               c->kstart[nxt].as<int32_t>(), c->kidx[nxt].as<int32_t>());
-  if (m > 0 && (flags & (PICGPU_STEP_DEPOSIT_CELLS | PICGPU_STEP_DEPOSIT_FACES)))
+  if (n > 0 && (flags & (PICGPU_STEP_DEPOSIT_CELLS | PICGPU_STEP_DEPOSIT_FACES)))
     PG_LAUNCH(c, "bucket_pid_order", nblocks(nkeys), kBlock, k_bucket_pid_order, nkeys, c->kstart[nxt].as<int32_t>(),
               c->kidx[nxt].as<int32_t>(), c->sw.pid.as<int64_t>());
-  unsigned long long h[C_NUM];
-  PG_CHECK(counters_read(c, h));
-  if (h[C_ERROR]) {
-    // the step stopped part-way: some particles carry this step's results,
-    // the others the state it started from, and the index is half built
-    c->swarm_invalid = true;
-    c->keyed = c->key_ordered = c->compact_pending = false;
-    set_error("degenerate cell face (|n| < 1e-12) met during the walk");
-    return PICGPU_E_DEGENERATE;
-  }
-  // every survivor of a fused step is located: the next index has no unlocated bucket
-  const int64_t live = (int64_t)h[C_LOCATED];
-  c->kcur = nxt;
-  c->kbegin_unloc = c->kend_unloc = live;
-  c->n_live = live;
-  c->stats.located = live;
-  c->stats.removed_lost = (int64_t)h[C_REMOVED_LOST];
-  c->stats.removed_out_of_bounds = (int64_t)h[C_REMOVED_OOB];
-  c->stats.lost = (int64_t)h[C_REMOVED_LOST];
-  c->stats.walk_iterations = (int64_t)h[C_WALK_ITERS];
-  c->stats.fp64_fallbacks = (int64_t)h[C_FALLBACKS];
   if (hist) PG_CHECK(hist_read(c));
+  // every survivor of a fused step is located: the next index has no unlocated bucket
+  c->kcur = nxt;
+  c->kbegin_unloc = c->kend_unloc = -1;  // known after the call's counters are read
   const int32_t *nidx = c->kidx[nxt].as<int32_t>();
   const int32_t *nstart = c->kstart[nxt].as<int32_t>();
   int r = dispatch_real(c, [&](auto *tag) -> int {
@@ -1039,7 +1103,8 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool last) {
                 nidx, c->sw.P.as<R>(), c->pavg.as<R>());
       c->have_pavg = true;
     }
-    if (flags & PICGPU_STEP_DEPOSIT_FACES) {
+    if (wu && faces) {
+      // weights and velocities were written: the step's deposition is a pass of its own
       for (int dir = 0; dir < 3; ++dir) PG_CHECK(c->face[dir].ensure(sizeof(R) * 4 * (size_t)nface(c, dir)));
       const TileMap xm{c->IM / TX + 1, c->JM / TY + 1, c->KM / TZ + 1};  // tiles over [0..IM]x[0..JM]x[0..KM]
       PG_LAUNCH(c, "deposit_faces", ntiles(xm), kBlock, k_deposit_tiles<R>, c->IM, c->JM, c->KM, tm, xm, nstart, nidx,
@@ -1053,12 +1118,100 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, bool last) {
   c->compact_pending = true;
   if (c->steps_since_compact >= kCompactEvery) {
     // every survivor of a fused step is located; at the call's last step the
-    // weights and statuses move too
-    PG_CHECK(compact_by_key(c, live, last ? GATHER_LAST : GATHER_STEP));
+    // weights and statuses move too; the live count is the new index's total
+    PG_CHECK(compact_by_key(c, n, last ? GATHER_LAST : GATHER_STEP, nstart + nkeys + 1));
     c->steps_since_compact = 0;
     c->compact_pending = false;
+    *compacted = true;
   }
   c->sorted = false;  // key order, not cell order
+  return PICGPU_OK;
+}
+
+// After the last step of a call (weights and velocities not kept between
+// steps): every particle's weights and velocity at its final position, and
+// the last step's face deposition.
+static int finish_step_call(picgpu_ctx *c, unsigned flags, unsigned long long *counters) {
+  const bool faces = (flags & PICGPU_STEP_DEPOSIT_FACES) && !(flags & PICGPU_STEP_FACES_OF_START);
+  const TileMap tm = tile_map(c);
+  const int64_t nt = ntiles(tm);
+  alignas(16) unsigned char fo_buf[sizeof(FaceOut<double>)];
+  if (faces) PG_CHECK(face_out(c, fo_buf));
+  int r = dispatch_real(c, [&](auto *tag) -> int {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    StepArgs<R> a;
+    fill_step_args<R>(c, a, 0.0, counters);
+    a.deposit = faces ? 1 : 0;
+    if (faces) a.fo = *reinterpret_cast<const FaceOut<R> *>(fo_buf);
+    constexpr bool EX = sizeof(R) == sizeof(double);
+    PG_LAUNCH(c, EX ? "step_finish_fp64" : "step_finish_fp32", nt, kBlock, (k_step_finish<R, EX>), a,
+              c->kidx[c->kcur].as<int32_t>(), c->kstart[c->kcur].as<int32_t>());
+    return PICGPU_OK;
+  });
+  PG_CHECK(r);
+  if (faces) {
+    PG_CHECK(launch_face_seams(c, fo_buf));
+    c->have_faces = true;
+  }
+  return PICGPU_OK;
+}
+
+// Fused steps s0 .. s0 + nb - 1 of a call of nsteps: queued, then one read of
+// their counters, which settles the live count, the array length after a
+// compaction and the statistics.
+static int step_batch(picgpu_ctx *c, double dt, unsigned flags, int s0, int nb, int nsteps) {
+  const bool fields = (flags & PICGPU_STEP_FIELDS) != 0;
+  PG_CHECK(c->step_counters.ensure(sizeof(unsigned long long) * kStepCounterWords * kStepBatch));
+  PG_HIP(hipMemsetAsync(c->step_counters.ptr, 0, sizeof(unsigned long long) * kStepCounterWords * (size_t)nb,
+                        c->stream));
+  unsigned long long *ctr = c->step_counters.as<unsigned long long>();
+  int64_t m0 = -1;  // entries of the index the batch starts from
+  int compacted_at = -1;
+  for (int b = 0; b < nb; ++b) {
+    const int s = s0 + b;
+    if (fields) PG_CHECK(eulerian_fields_async(c));
+    PG_CHECK(do_center_to_corner(c));
+    bool compacted = false;
+    int64_t entries = 0;
+    PG_CHECK(do_step_fused(c, dt, flags, s, s == nsteps - 1, ctr + (size_t)b * kStepCounterWords, &compacted,
+                           &entries));
+    if (b == 0) m0 = entries;
+    if (compacted) compacted_at = b;
+  }
+  if (!fields && s0 + nb == nsteps) PG_CHECK(finish_step_call(c, flags, ctr + (size_t)(nb - 1) * kStepCounterWords));
+  std::vector<unsigned long long> h(kStepCounterWords * (size_t)nb);
+  PG_HIP(hipMemcpyAsync(h.data(), ctr, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, c->stream));
+  PG_HIP(hipStreamSynchronize(c->stream));
+  auto sum = [&](int b, int k) {
+    unsigned long long v = 0;
+    for (int st = 0; st < kCounterStripes; ++st) v += h[(size_t)b * kStepCounterWords + (size_t)st * C_NUM + k];
+    return (int64_t)v;
+  };
+  bool err = false;
+  int64_t m = m0;
+  for (int b = 0; b < nb; ++b) {
+    c->stats.particle_steps += m;
+    m = sum(b, C_LOCATED);
+    err = err || sum(b, C_ERROR) != 0;
+  }
+  const int l = nb - 1;
+  c->stats.located = m;
+  c->stats.removed_lost = sum(l, C_REMOVED_LOST);
+  c->stats.removed_out_of_bounds = sum(l, C_REMOVED_OOB);
+  c->stats.lost = c->stats.removed_lost;
+  c->stats.walk_iterations = sum(l, C_WALK_ITERS);
+  c->stats.fp64_fallbacks = sum(l, C_FALLBACKS);
+  if (err) {
+    // a step stopped part-way: some particles carry that step's results, the
+    // others the state it started from; later steps ran on that mix
+    c->swarm_invalid = true;
+    c->keyed = c->key_ordered = c->compact_pending = false;
+    set_error("degenerate cell face (|n| < 1e-12) met during the walk");
+    return PICGPU_E_DEGENERATE;
+  }
+  c->n_live = m;
+  c->kbegin_unloc = c->kend_unloc = m;
+  if (compacted_at >= 0) c->n = sum(compacted_at, C_LOCATED);
   return PICGPU_OK;
 }
 
@@ -1159,14 +1312,12 @@ int picgpu_step(picgpu_ctx *c, double dt, int nsteps, unsigned flags) {
   if (!c->compact_pending) c->steps_since_compact = 0;
   const bool fields = (flags & PICGPU_STEP_FIELDS) != 0;
   if (fields) PG_CHECK(eulerian_fields_begin(c));
-  for (int s = 0; s < nsteps; ++s) {
+  if (!(flags & PICGPU_STEP_STAGED)) {
+    for (int s0 = 0; s0 < nsteps; s0 += kStepBatch)
+      PG_CHECK(step_batch(c, dt, flags, s0, std::min(kStepBatch, nsteps - s0), nsteps));
+  }
+  for (int s = 0; s < nsteps && (flags & PICGPU_STEP_STAGED); ++s) {
     if (fields) PG_CHECK(eulerian_fields_async(c));
-    if (!(flags & PICGPU_STEP_STAGED)) {
-      PG_CHECK(do_center_to_corner(c));
-      PG_CHECK(do_step_fused(c, dt, flags, s == nsteps - 1));
-      c->n_live = c->stats.located;
-      continue;
-    }
     c->stats.particle_steps += c->n;
     PG_CHECK(do_center_to_corner(c));
     if (flags & PICGPU_STEP_RK2) {

@@ -153,6 +153,8 @@ struct picgpu_ctx {
   picgpu::DevBuf kslot;      // int32 [n] slot of each particle in its next bucket
   picgpu::DevBuf kbase;      // int32 [nkeys] a tile's base in each of its own buckets
   picgpu::DevBuf defer;      // int32 [n] particles deferred to the FP64 kernel, then their number
+  picgpu::DevBuf step_counters;  // uint64 [steps of a batch][kCounterStripes][C_NUM]
+  picgpu::DevBuf seams;      // real: halves of the faces between two tiles (FaceOut)
   int kcur = 0;              // index in use
   bool keyed = false;        // kidx[kcur] / kstart[kcur] describe the current swarm
   bool key_ordered = false;  // the swarm arrays are (nearly) in key order: compacted since the last change

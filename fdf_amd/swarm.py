@@ -184,12 +184,15 @@ class PicGpu:
         L.check(self.lib.picgpu_set_particle_volume(self.h, particle_volume))
 
     def AdvanceSimulation(self, dt: float, nsteps: int, rk2: bool = False, deposit_cells: bool = True,
-                          deposit_faces: bool = False, staged: bool = False, fields: bool = False):
+                          deposit_faces: bool = False, staged: bool = False, fields: bool = False,
+                          faces_of_start: bool = False):
         """fields=True: every step first prepares Ucat (boundaries, Contra2Cart,
-        boundaries), as SetEulerianFields does for the steps after the first."""
+        boundaries), as SetEulerianFields does for the steps after the first.
+        faces_of_start=True: the faces hold the deposition of the state at the
+        start of the call's last step (PICGPU_STEP_FACES_OF_START)."""
         flags = (L.STEP_RK2 if rk2 else 0) | (L.STEP_DEPOSIT_CELLS if deposit_cells else 0) | \
                 (L.STEP_DEPOSIT_FACES if deposit_faces else 0) | (L.STEP_STAGED if staged else 0) | \
-                (L.STEP_FIELDS if fields else 0)
+                (L.STEP_FIELDS if fields else 0) | (L.STEP_FACES_OF_START if faces_of_start else 0)
         L.check(self.lib.picgpu_step(self.h, dt, nsteps, flags))
 
     # ---- Eulerian field preparation ----

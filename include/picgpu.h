@@ -86,6 +86,10 @@ extern "C" {
                                          Contra2Cart, boundaries); needs picgpu_boundary_create */
 #define PICGPU_STEP_STAGED 0x100      /* run the stage kernels one by one (reference stage order)
                                          instead of the fused one-pass step; same results */
+#define PICGPU_STEP_FACES_OF_START 0x20 /* with DEPOSIT_FACES, fused path, nsteps >= 2: the faces hold
+                                         the deposition of the state at the start of the call's last
+                                         step (each step deposits the state it starts from; the call's
+                                         end state is then not deposited) */
 
 typedef struct picgpu_ctx picgpu_ctx;
 

@@ -421,12 +421,14 @@ def test_timing_select(swarm_cls):
         g.synchronize()
         ms, n = g.timing_query("step_fused_fp32")
         assert n == 3 and ms > 0
-        assert g.timing_query("deposit_faces")[1] == 0 and g.timing_query("center_to_corner")[1] == 0
+        assert g.timing_query("step_finish_fp32")[1] == 0 and g.timing_query("center_to_corner")[1] == 0
         g.timing_select("")
         g.timing_reset()
         g.AdvanceSimulation(d["dt"], 2, rk2=True, deposit_faces=True)
         g.synchronize()
-        assert g.timing_query("step_fused_fp32")[1] == 2 and g.timing_query("deposit_faces")[1] == 2
+        # the second step deposits the first step's state, the call's finishing pass the second's
+        assert g.timing_query("step_fused_fp32")[1] == 2 and g.timing_query("step_finish_fp32")[1] == 1
+        assert g.timing_query("face_seams")[1] == 2
         g.timing(False)
 
 
@@ -453,3 +455,57 @@ def test_long_fused_run_matches_staged(swarm_cls, precision):
     assert np.array_equal(fa, fb)
     assert sa["particle_steps"] == sb["particle_steps"]
     assert len(a["pid"]) < 60000  # some particles left the domain
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", ["warped", "crowded"])
+def test_steps_deposit_the_state_they_start_from(swarm_cls, oracle, case, precision):
+    """Inside a call every step deposits the state it starts from (the
+    previous step's deposition, made while the step walks its particles);
+    with faces_of_start the faces keep the one the call's last step made.
+    It equals the oracle's deposition of the state one step before the end:
+    bit for bit in FP64, within 1e-5 in FP32 (of the FP32 run's own state).
+    The crowded case has thousands of particles per tile, more than the
+    kernel stages in LDS."""
+    _say("deposit_start_state", case, precision)
+    if case == "warped":
+        d = W.config(5, scale=0.02)
+        d["pos"] = d["pos"][:30000]
+        d["dt"] *= 3
+    else:
+        IM, JM, KM = 16, 16, 8
+        coor = W.perturbed_cube(IM, JM, KM, seed=31, amp=0.15)
+        ucat = W.ucat_random_fourier(coor, IM, JM, KM, seed=32)
+        d = {"IM": IM, "JM": JM, "KM": KM, "coor": coor, "ucat": ucat,
+             "pos": 0.05 + 0.9 * np.random.default_rng(33).random((40000, 3)),
+             "dt": 1.5 / (np.abs(ucat).max() * max(IM, JM, KM))}
+    IM, JM, KM = d["IM"], d["JM"], d["KM"]
+    nsteps = 4
+
+    def run(n, faces_of_start):
+        with swarm_cls(precision=precision) as g:
+            g.upload_grid(IM, JM, KM, d["coor"])
+            g.upload_ucat(d["ucat"])
+            g.upload_particles(d["pos"])
+            g.LocateAllParticlesInGrid()
+            g.InterpolateAllFieldsToSwarm()
+            g.AdvanceSimulation(d["dt"], n, rk2=True, deposit_cells=False, deposit_faces=True,
+                                faces_of_start=faces_of_start)
+            return _by_pid(g.download_particles()), [g.field(f) for f in (L.FIELD_IFACE, L.FIELD_JFACE,
+                                                                          L.FIELD_KFACE)]
+
+    before, _ = run(nsteps - 1, False)
+    after, faces = run(nsteps, True)
+    assert len(after["pid"]) < len(before["pid"]) or case == "warped"
+    ref = oracle.deposit_faces(IM, JM, KM, before["cell"], before["weight"], before["vel"], before["pid"])
+    if precision == "fp64":
+        s, _ = _run_oracle_steps(oracle, d, nsteps - 1, True)
+        for k in ("pid", "cell", "pos", "vel", "weight"):
+            assert np.array_equal(before[k], s[k]), k
+        for a, b in zip(faces, ref):
+            assert np.array_equal(a, b)
+    else:
+        for a, b in zip(faces, ref):
+            a4, b4 = a.reshape(-1, 4), b.reshape(-1, 4)
+            assert np.abs(a4[:, 0] - b4[:, 0]).max() <= FP32_TOL * max(1.0, np.abs(b4[:, 0]).max())
+            assert np.abs(a4[:, 1:] - b4[:, 1:]).max() <= FP32_TOL * max(1.0, np.abs(b4[:, 1:]).max())
