@@ -44,6 +44,34 @@ __device__ __forceinline__ int xcd_tile(int b, int nt) {
   return x * q + (x < r ? x : r) + k;
 }
 
+// Tile at position m of the brick order: bricks of kBrickI x kBrickJ x
+// kBrickK tiles in raster order (i fastest), the tiles of a brick in raster
+// order, partial bricks at the high ends of each axis.  Positions that are
+// close hold tiles that are close in all three directions, so the halo
+// nodes that neighbouring tiles stage in LDS are read into the L2 once.
+constexpr int kBrickI = 8, kBrickJ = 4, kBrickK = 4;
+__host__ __device__ __forceinline__ int brick_tile(int m, const TileMap &t) {
+  const int slab = kBrickK * t.NTJ * t.NTI;  // tiles of a full layer of bricks
+  const int bk = m / slab;
+  int r = m - bk * slab;
+  const int ek = t.NTK - bk * kBrickK < kBrickK ? t.NTK - bk * kBrickK : kBrickK;
+  const int row = kBrickJ * t.NTI * ek;  // tiles of a full row of bricks in that layer
+  const int bj = r / row;
+  r -= bj * row;
+  const int ej = t.NTJ - bj * kBrickJ < kBrickJ ? t.NTJ - bj * kBrickJ : kBrickJ;
+  const int full = kBrickI * ej * ek;  // tiles of a full brick in that row
+  const int bi = r / full;
+  r -= bi * full;
+  const int ei = t.NTI - bi * kBrickI < kBrickI ? t.NTI - bi * kBrickI : kBrickI;
+  const int li = r % ei, lj = (r / ei) % ej, lk = r / (ei * ej);
+  return ((bk * kBrickK + lk) * t.NTJ + bj * kBrickJ + lj) * t.NTI + bi * kBrickI + li;
+}
+// Tile of workgroup b of a tiled step launch (one workgroup per tile): each
+// XCD takes a contiguous share of the brick order.
+__device__ __forceinline__ int step_tile(int b, const TileMap &t) {
+  return brick_tile(xcd_tile(b, t.NTI * t.NTJ * t.NTK), t);
+}
+
 __device__ __forceinline__ int32_t cell_key(const TileMap &t, int i, int j, int k) {
   const int ti = i / TX, tj = j / TY, tk = k / TZ;
   const int li = i - ti * TX, lj = j - tj * TY, lk = k - tk * TZ;
@@ -62,6 +90,32 @@ struct FaceOut {
   R *face[3];     // i-faces [KM][JM][IM+1], j [KM][JM+1][IM], k [KM+1][JM][IM], 4 reals each
   R *lo[3], *hi[3];
 };
+
+// A face's four reals (count, momentum) as one vector store / load.
+template <typename R>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Vec4<double> {
+  using type = double __attribute__((ext_vector_type(4)));
+};
+template <typename R>
+__device__ __forceinline__ void store4(R *dst, R a, R b, R c, R d) {
+  using V = typename Vec4<R>::type;
+  V v;
+  v.x = a;
+  v.y = b;
+  v.z = c;
+  v.w = d;
+  *reinterpret_cast<V *>(dst) = v;
+}
+template <typename R>
+__device__ __forceinline__ typename Vec4<R>::type load4(const R *src) {
+  return *reinterpret_cast<const typename Vec4<R>::type *>(src);
+}
 
 __device__ __forceinline__ int64_t seam_entry(int d, int m, int i, int j, int k, int IM, int JM, int KM) {
   if (d == 0) return ((int64_t)(m - 1) * KM + k) * JM + j;
@@ -157,7 +211,7 @@ struct PlaneSrc {
   const R *corner;
   const typename PlanesOf<R>::type *planes;
   lds_ptr<R> sk;   // TILE: [BZ][BY][BX][3]
-  int i0, j0, k0;  // TILE: node origin of the staged box
+  int i0, j0, k0;  // TILE: node origin of the staged box (the tile's first cell - 1)
   template <bool EXACT>
   __device__ __forceinline__ int eval(int i, int j, int k, const double x[3], double ts, R d[6],
                                       unsigned long long &fb) const {
@@ -486,20 +540,15 @@ __device__ __forceinline__ void flush_tally(StepTally &t, unsigned long long *co
 // in each bucket (kbase) is added when the next index is built.
 constexpr int32_t kLocalSlot = 0x40000000;
 
-// Results of the first OutCap<R> particles of a tile are staged in LDS and
-// written out together at the end of the workgroup, in index order: the
-// lanes finish their particles at different times, and writing each result
-// as it comes scatters partial cache lines over HBM.
-template <typename R>
-struct OutCap {
-  static constexpr int value = sizeof(R) == 4 ? 448 : 384;
-};
 // Start states (weights and velocity) of the first DepCap<R> particles of a
 // tile are kept in LDS for the tile's face sums; those of later particles
-// go to the w and vel arrays.
+// go to the w and vel arrays.  The FP32 step kernel's registers hold it to
+// four workgroups per CU, which leaves LDS for 896 entries (a tile holds
+// about 512 particles at two per cell); the FP64 figure keeps three
+// workgroups' LDS within the CU's 160 KiB.
 template <typename R>
 struct DepCap {
-  static constexpr int value = sizeof(R) == 4 ? 512 : 352;
+  static constexpr int value = sizeof(R) == 4 ? 896 : 352;
 };
 enum OutKind : int8_t {
   OUT_NONE = 0,   // deferred to k_step_deferred: nothing is written
@@ -509,22 +558,50 @@ enum OutKind : int8_t {
   OUT_OOB = 4,    // outside the bounding box: position, cell, weights, removed
   OUT_LOC = 5     // located: position, cell, weights, velocity, slot
 };
-// WU: weights and velocity are written (otherwise the next step recomputes them)
-template <typename R, bool WU>
-struct OutStage {
-  static constexpr int N = OutCap<R>::value;
-  static constexpr int NW = WU ? N : 1;
-  double pos[N][3];
-  R vel[NW][3];
-  R w[NW][3];
-  int32_t cell[N];
-  int32_t slot[N];
-  int8_t kind[N];
+
+// The staging loads of a tile (corner velocities of the staged box), held
+// in registers between issue() and store() so that a kernel can issue its
+// first particle loads in between.
+template <typename R>
+struct TileLoads {
+  static constexpr int NU = (BN + kBlock - 1) / kBlock;  // staged nodes per thread
+  R cv[NU][3];
+  int64_t node[NU];
+  template <class Src>
+  __device__ __forceinline__ void issue(const StepArgs<R> &a, const Src &src) {
+#pragma unroll
+    for (int m = 0; m < NU; ++m) {
+      const int u = threadIdx.x + m * kBlock;
+      const int bx = u % BX, by = (u / BX) % BY, bz = u / (BX * BY);
+      const int ni = src.i0 + bx, nj = src.j0 + by, nk = src.k0 + bz;
+      node[m] = -1;
+      if (u < BN && ni >= 0 && ni <= a.g.IM && nj >= 0 && nj <= a.g.JM && nk >= 0 && nk <= a.g.KM)
+        node[m] = ((int64_t)nk * (a.g.JM + 1) + nj) * (a.g.IM + 1) + ni;
+    }
+#pragma unroll
+    for (int m = 0; m < NU; ++m)
+      if (node[m] >= 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) cv[m][c] = a.corner[3 * node[m] + c];
+      }
+  }
+  __device__ __forceinline__ void store(R *sk) const {
+#pragma unroll
+    for (int m = 0; m < NU; ++m)
+      if (node[m] >= 0) {
+        const int u = threadIdx.x + m * kBlock;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sk[3 * u + c] = cv[m][c];
+      }
+  }
 };
 
+// A particle's result.  WU: weights and velocity are written (otherwise the
+// next step recomputes them).  wslot false: the slot is written later.
 template <typename R, bool WU>
 __device__ __forceinline__ void emit_global(const StepArgs<R> &a, int64_t p, int kind, const double x[3],
-                                            int32_t cell, const R w[3], const R u[3], int32_t slot) {
+                                            int32_t cell, const R w[3], const R u[3], int32_t slot,
+                                            bool wslot = true) {
   if (kind == OUT_NONE) return;
   if (kind != OUT_SLOT) {
     a.pos[3 * p] = x[0];
@@ -545,7 +622,7 @@ __device__ __forceinline__ void emit_global(const StepArgs<R> &a, int64_t p, int
     a.vel[3 * p + 1] = u[1];
     a.vel[3 * p + 2] = u[2];
   }
-  a.slot[p] = slot;
+  if (wslot) a.slot[p] = slot;
 }
 
 // Start state of a located particle at x0 in its cell (the record index lc,
@@ -643,24 +720,22 @@ __device__ __forceinline__ void tile_face_sums(const StepArgs<R> &a, int t, int3
     R *face = a.fo.face[d];
     const int64_t fl = face_index(d, i, j, k, IM, JM);
     if (l3[d] > 0) {  // lower neighbour in the tile
-#pragma unroll
-      for (int v = 0; v < 4; ++v) face[4 * fl + v] = xch[v][l - STR[d]] + hl[d][v].value();
+      store4<R>(face + 4 * fl, xch[0][l - STR[d]] + hl[d][0].value(), xch[1][l - STR[d]] + hl[d][1].value(),
+                xch[2][l - STR[d]] + hl[d][2].value(), xch[3][l - STR[d]] + hl[d][3].value());
     } else if (g3[d] == 0) {  // the domain's first face: no lower cell
-#pragma unroll
-      for (int v = 0; v < 4; ++v) face[4 * fl + v] = (R)0 + hl[d][v].value();
+      store4<R>(face + 4 * fl, (R)0 + hl[d][0].value(), (R)0 + hl[d][1].value(), (R)0 + hl[d][2].value(),
+                (R)0 + hl[d][3].value());
     } else {
       const int64_t se = seam_entry(d, g3[d] / T3[d], i, j, k, IM, JM, KM);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) a.fo.lo[d][4 * se + v] = hl[d][v].value();
+      store4<R>(a.fo.lo[d] + 4 * se, hl[d][0].value(), hl[d][1].value(), hl[d][2].value(), hl[d][3].value());
     }
     if (g3[d] + 1 == dims[d]) {  // the domain's last face: no upper cell
       const int64_t fu = face_index(d, i + (d == 0), j + (d == 1), k + (d == 2), IM, JM);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) face[4 * fu + v] = hh[d][v].value() + (R)0;
+      store4<R>(face + 4 * fu, hh[d][0].value() + (R)0, hh[d][1].value() + (R)0, hh[d][2].value() + (R)0,
+                hh[d][3].value() + (R)0);
     } else if (l3[d] == T3[d] - 1) {  // upper neighbour in the next tile
       const int64_t se = seam_entry(d, (g3[d] + 1) / T3[d], i, j, k, IM, JM, KM);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) a.fo.hi[d][4 * se + v] = hh[d][v].value();
+      store4<R>(a.fo.hi[d] + 4 * se, hh[d][0].value(), hh[d][1].value(), hh[d][2].value(), hh[d][3].value());
     }
   }
 }
@@ -678,8 +753,12 @@ constexpr int kSlowCap = 256;
 // corrector both accept the particle's own cell at the first evaluation,
 // every face certified in FP32 mode -- is stepped in straight-line code; one
 // plane record serves the start state, the predictor and the corrector.
-// With WU = false the step ends there: the next step recomputes the
-// weights and velocity at the new position.  Every other particle (a walk
+// A lane's next particle's state (and the index entry of the one after) is
+// requested before the current particle is stepped, and the first ones with
+// the staging loads.  Results are written as they come: the lanes of a
+// wavefront finish fast-pass particles together, at consecutive index
+// positions.  With WU = false the step ends there: the next step recomputes
+// the weights and velocity at the new position.  Every other particle (a walk
 // that leaves the cell, an uncertified face, a degenerate cell, a particle
 // that is not located at the start of the step) is queued for the general
 // loop.
@@ -707,39 +786,37 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
                                                        const int32_t *__restrict__ kstart,
                                                        int32_t *__restrict__ kbase) {
   static_assert(TS == kBlock, "one thread per key of the tile");
-  constexpr int CAP = OutCap<R>::value;
   constexpr int CAPD = WU ? 1 : DepCap<R>::value;
   __shared__ R sk[BN * 3];
   __shared__ int32_t scount[TS];
-  __shared__ union {
-    OutStage<R, WU> so;
-    R xch[4][TS];
-  } su;
+  __shared__ R xch[4][TS];  // the face-sum exchange
   __shared__ R sd[CAPD][6];
   __shared__ int32_t shist[kWalkBins];
   __shared__ int32_t squeue[kSlowCap];
   __shared__ int32_t sqn;
-  auto &so = su.so;
-  const int t = blockIdx.x;
+  const int t = step_tile(blockIdx.x, a.tm);
   const int32_t s = kstart[(int64_t)t * TS], e = kstart[(int64_t)(t + 1) * TS];
   const bool dep = !WU && a.deposit;
   const bool recompute = !WU && a.u0_recompute;
   if (s == e && !dep) return;  // empty tile (uniform over the block)
   const int ti = t % a.tm.NTI, tj = (t / a.tm.NTI) % a.tm.NTJ, tk = t / (a.tm.NTI * a.tm.NTJ);
   PlaneSrc<R, true, !EXACT> src{a.g, a.corner, (const typename PlanesOf<R>::type *)a.planes, (lds_ptr<R>)sk,
-                               ti * TX - 1, tj * TY - 1, tk * TZ - 1};
+                                ti * TX - 1, tj * TY - 1, tk * TZ - 1};
   const int32_t B = (int32_t)blockDim.x;
+
+  // The first fast-pass particle of this lane and the one after it; their
+  // loads are issued with the staging loads, before the first barrier.
+  int32_t q = s + (int32_t)threadIdx.x;
+  int32_t p = q < e ? idx[q] : -1;
+  int32_t pn = q + B < e ? idx[q + B] : -1;
+  PState<R> ps;
   if (s < e) {
-    const int64_t sy = a.g.IM + 1, sz = (int64_t)(a.g.JM + 1) * (a.g.IM + 1);
-    for (int u = threadIdx.x; u < BN; u += blockDim.x) {
-      const int bx = u % BX, by = (u / BX) % BY, bz = u / (BX * BY);
-      const int ni = src.i0 + bx, nj = src.j0 + by, nk = src.k0 + bz;
-      if (ni >= 0 && ni <= a.g.IM && nj >= 0 && nj <= a.g.JM && nk >= 0 && nk <= a.g.KM) {
-        const int64_t node = (int64_t)nk * sz + (int64_t)nj * sy + ni;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) sk[3 * u + c] = a.corner[3 * node + c];
-      }
-    }
+    // every staging load is issued before the first particle's state, and
+    // that before the first LDS store
+    TileLoads<R> tl;
+    tl.issue(a, src);
+    if (p >= 0) load_state(a, p, ps, !recompute);
+    tl.store(sk);
   }
   scount[threadIdx.x] = 0;
   if (threadIdx.x < kWalkBins) shist[threadIdx.x] = 0;
@@ -748,31 +825,9 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
 
   StepTally tally;
   const double h = 0.5 * a.dt;
-  // a finished particle's result: staged in LDS (index order) or written now
-  auto finish = [&](int32_t q, int64_t p, int kind, const double x[3], int32_t cell, const R w[3], const R u[3],
-                    int32_t slot) {
-    const int32_t lq = q - s;
-    if (lq < CAP) {
-      so.kind[lq] = (int8_t)kind;
-      so.pos[lq][0] = x[0];
-      so.pos[lq][1] = x[1];
-      so.pos[lq][2] = x[2];
-      so.cell[lq] = cell;
-      so.slot[lq] = slot;
-      if constexpr (WU) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          so.w[lq][c] = w[c];
-          so.vel[lq][c] = u[c];
-        }
-      }
-    } else {
-      emit_global<R, WU>(a, p, kind, x, cell, w, u, slot);
-    }
-  };
   // the start state's deposition entry of index position q
-  auto stage_dep = [&](int32_t q, int64_t p, const R w0[3], const R u0[3]) {
-    const int32_t lq = q - s;
+  auto stage_dep = [&](int32_t qq, int64_t pp, const R w0[3], const R u0[3]) {
+    const int32_t lq = qq - s;
     if (lq < CAPD) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -782,26 +837,28 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
     } else {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        a.w[3 * p + c] = w0[c];
-        a.vel[3 * p + c] = u0[c];
+        a.w[3 * pp + c] = w0[c];
+        a.vel[3 * pp + c] = u0[c];
       }
     }
   };
 
   // ---- fast pass ----
-  for (int32_t q = s + (int32_t)threadIdx.x; q < e; q += B) {
-    const int32_t p = idx[q];
-    PState<R> ps;
-    load_state(a, p, ps, !recompute);
+  while (p >= 0) {
+    // the lane's next particle but one, and the next particle's state
+    const int32_t q2 = q + 2 * B;
+    const int32_t p2 = q2 < e ? idx[q2] : -1;
+    PState<R> psn;
+    if (pn >= 0) load_state(a, pn, psn, !recompute);
     bool fast = ps.st == ST_LOCATED && ps.c0 >= 0;
     if (fast) {
       const int32_t lc = ps.c0;
       int ci, cj, ck;
       unpack_cell(lc, cell_idx(a.g), ci, cj, ck);
       // FP32: the record, read once into registers, serves every evaluation;
-      // FP64: its 304 bytes are read where they are used
+      // FP64: its 304 bytes are read from global memory where they are used
       CellPlanes32 rec32;
-      if constexpr (!EXACT) rec32 = static_cast<const CellPlanes32 *>(a.planes)[lc];
+      if constexpr (!EXACT) rec32 = src.planes[lc];
       auto dist = [&](const double x[3], R d[6]) -> bool {  // certified distances
         if constexpr (EXACT)
           return eval_planes_exact(src.planes[lc], x, d);
@@ -853,7 +910,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
         } else {
           tally.oob++;
         }
-        finish(q, p, in ? OUT_LOC : OUT_OOB, x1, lc, w1, u1, slot);
+        emit_global<R, WU>(a, p, in ? OUT_LOC : OUT_OOB, x1, lc, w1, u1, slot);
       }
     }
     if (!fast) {
@@ -868,9 +925,12 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
 #pragma unroll
           for (int c = 0; c < 3; ++c) a.vel[3 * p + c] = ps.u0[c];
         }
-        finish(q, p, OUT_NONE, ps.x0, -1, ps.u0, ps.u0, -1);
       }
     }
+    q += B;
+    p = pn;
+    pn = p2;
+    ps = psn;
   }
   __syncthreads();
 
@@ -878,11 +938,10 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
   const int32_t nq = sqn < kSlowCap ? sqn : kSlowCap;
   int32_t r = (int32_t)threadIdx.x;  // queue entry of the current particle
   auto qpos = [&](int32_t rr) -> int32_t { return rr < nq ? squeue[rr] : -1; };
-  int32_t q = qpos(r);
-  int32_t p = q >= 0 ? idx[q] : -1;
+  q = qpos(r);
+  p = q >= 0 ? idx[q] : -1;
   int32_t qn = qpos(r + B);
-  int32_t pn = qn >= 0 ? idx[qn] : -1;
-  PState<R> ps;
+  pn = qn >= 0 ? idx[qn] : -1;
   if (p >= 0) load_state(a, p, ps, !recompute);
   PState<R> pnext;  // state of particle pn, requested one particle ahead
   int32_t pnn = -1;
@@ -894,7 +953,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
   int32_t lc = 0;  // ci + IM (cj + JM ck)
   CycleGuard cyc;
   R wv[3] = {0, 0, 0};
-  int32_t fq = -1, fv = 0;  // a slot taken in the global histogram, stored at the next one (or the end)
+  int32_t fp = -1, fv = 0;  // a slot taken in the global histogram, stored at the next one (or the end)
 
   auto begin = [&]() {
     // request the next particle's state, then set up the walk of p
@@ -927,13 +986,6 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
     else
       ph = RK2 ? 1 : 2;
   };
-  // the slot of the particle at index position qq (staged or global)
-  auto put_slot = [&](int32_t qq, int32_t v) {
-    if (qq - s < CAP)
-      so.slot[qq - s] = v;
-    else
-      a.slot[idx[qq]] = v;
-  };
   if (p >= 0) begin();
 
   while (ph != 0) {
@@ -960,6 +1012,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
     // particle p is done: outcome rw at position xw (x0 when LOST at start)
     int kind;
     int32_t cell = -1, slot = -1;
+    bool slot_later = false;
     if (rw == W_DEFER) {
       // stepped again from its start state by k_step_deferred, which reads
       // the start velocity from vel
@@ -995,9 +1048,10 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
           if (key / TS == t) {
             slot = atomicAdd(&scount[key - t * TS], 1) | kLocalSlot;
           } else {
-            if (fq >= 0) put_slot(fq, fv);
-            fq = q;
+            if (fp >= 0) a.slot[fp] = fv;
+            fp = p;
             fv = atomicAdd(&a.count_next[key], 1);
+            slot_later = true;
           }
         } else {
           tally.oob++;
@@ -1005,7 +1059,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
         }
       }
     }
-    finish(q, p, kind, xw, cell, wv, ui, slot);
+    emit_global<R, WU>(a, p, kind, xw, cell, wv, ui, slot, !slot_later);
     // next particle
     r += B;
     q = qn;
@@ -1018,25 +1072,9 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
       begin();
     }
   }
-  if (fq >= 0) put_slot(fq, fv);
+  if (fp >= 0) a.slot[fp] = fv;
   flush_tally(tally, a.counters);
   __syncthreads();
-  // staged results, in index order
-  const int32_t nst = e - s < CAP ? e - s : CAP;
-  for (int32_t lq = threadIdx.x; lq < nst; lq += B) {
-    const int kind = so.kind[lq];
-    if (kind == OUT_NONE) continue;
-    const double x[3] = {so.pos[lq][0], so.pos[lq][1], so.pos[lq][2]};
-    R w[3] = {0, 0, 0}, u[3] = {0, 0, 0};
-    if constexpr (WU) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        w[c] = so.w[lq][c];
-        u[c] = so.vel[lq][c];
-      }
-    }
-    emit_global<R, WU>(a, (int64_t)idx[s + lq], kind, x, so.cell[lq], w, u, so.slot[lq]);
-  }
   const int32_t cnt = scount[threadIdx.x];
   if (cnt > 0) {
     const int64_t key = (int64_t)t * TS + threadIdx.x;
@@ -1046,7 +1084,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
     atomicAdd(&a.walk_hist[(blockIdx.x & (kCounterStripes - 1)) * kWalkBins + threadIdx.x],
               (unsigned long long)shist[threadIdx.x]);
   if constexpr (!WU) {
-    if (dep) tile_face_sums<R, CAPD>(a, t, s, idx, kstart, sd, su.xch);
+    if (dep) tile_face_sums<R, CAPD>(a, t, s, idx, kstart, sd, xch);
   }
 }
 
@@ -1061,36 +1099,47 @@ __global__ __launch_bounds__(kBlock) void k_step_finish(StepArgs<R> a, const int
   __shared__ R sk[BN * 3];
   __shared__ R xch[4][TS];
   __shared__ R sd[CAPD][6];
-  const int t = blockIdx.x;
+  const int t = step_tile(blockIdx.x, a.tm);
   const int32_t s = kstart[(int64_t)t * TS], e = kstart[(int64_t)(t + 1) * TS];
   if (s == e && !a.deposit) return;
   const int ti = t % a.tm.NTI, tj = (t / a.tm.NTI) % a.tm.NTJ, tk = t / (a.tm.NTI * a.tm.NTJ);
   PlaneSrc<R, true, !EXACT> src{a.g, a.corner, (const typename PlanesOf<R>::type *)a.planes, (lds_ptr<R>)sk,
-                               ti * TX - 1, tj * TY - 1, tk * TZ - 1};
+                                ti * TX - 1, tj * TY - 1, tk * TZ - 1};
+  const int32_t B = (int32_t)blockDim.x;
   if (s < e) {
-    const int64_t sy = a.g.IM + 1, sz = (int64_t)(a.g.JM + 1) * (a.g.IM + 1);
-    for (int u = threadIdx.x; u < BN; u += blockDim.x) {
-      const int bx = u % BX, by = (u / BX) % BY, bz = u / (BX * BY);
-      const int ni = src.i0 + bx, nj = src.j0 + by, nk = src.k0 + bz;
-      if (ni >= 0 && ni <= a.g.IM && nj >= 0 && nj <= a.g.JM && nk >= 0 && nk <= a.g.KM) {
-        const int64_t node = (int64_t)nk * sz + (int64_t)nj * sy + ni;
+    int32_t q = s + (int32_t)threadIdx.x;
+    int32_t p = q < e ? idx[q] : -1;
+    int32_t pn = q + B < e ? idx[q + B] : -1;
+    int32_t lc = 0;
+    double x0[3];
+    TileLoads<R> tl;
+    tl.issue(a, src);
+    if (p >= 0) {
+      lc = a.cell[p];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) sk[3 * u + c] = a.corner[3 * node + c];
-      }
+      for (int c = 0; c < 3; ++c) x0[c] = a.pos[3 * p + c];
     }
+    tl.store(sk);
     __syncthreads();
-    for (int32_t q = s + (int32_t)threadIdx.x; q < e; q += (int32_t)blockDim.x) {
-      const int64_t p = idx[q];
-      const int32_t lc = a.cell[p];
-      const double x0[3] = {a.pos[3 * p], a.pos[3 * p + 1], a.pos[3 * p + 2]};
+    while (p >= 0) {
+      // the lane's next particle but one, and the next particle's state
+      const int32_t q2 = q + 2 * B;
+      const int32_t p2 = q2 < e ? idx[q2] : -1;
+      int32_t lcn = 0;
+      double xn[3];
+      if (pn >= 0) {
+        lcn = a.cell[pn];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xn[c] = a.pos[3 * pn + c];
+      }
       int ci, cj, ck;
       unpack_cell(lc, cell_idx(a.g), ci, cj, ck);
       R w0[3], u0[3];
       start_state<R, EXACT>(a, src, p, lc, ci, cj, ck, x0, w0, u0);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        a.w[3 * p + c] = w0[c];
-        a.vel[3 * p + c] = u0[c];
+        a.w[3 * (int64_t)p + c] = w0[c];
+        a.vel[3 * (int64_t)p + c] = u0[c];
       }
       if (a.deposit && q - s < CAPD) {
 #pragma unroll
@@ -1099,6 +1148,12 @@ __global__ __launch_bounds__(kBlock) void k_step_finish(StepArgs<R> a, const int
           sd[q - s][3 + c] = u0[c];
         }
       }
+      q += B;
+      p = pn;
+      pn = p2;
+      lc = lcn;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) x0[c] = xn[c];
     }
   }
   if (a.deposit) {
@@ -1141,8 +1196,8 @@ __global__ __launch_bounds__(kBlock) void k_face_seams(int IM, int JM, int KM, i
   const int64_t f = face_index(d, i, j, k, IM, JM);
   R *face = fo.face[d];
   const R *lo = fo.lo[d] + 4 * q, *hi = fo.hi[d] + 4 * q;
-#pragma unroll
-  for (int v = 0; v < 4; ++v) face[4 * f + v] = hi[v] + lo[v];
+  const auto vh = load4<R>(hi), vl = load4<R>(lo);
+  store4<R>(face + 4 * f, vh.x + vl.x, vh.y + vl.y, vh.z + vl.z, vh.w + vl.w);
 }
 
 // Particles deferred by k_step_tiled, stepped from their start state with the
