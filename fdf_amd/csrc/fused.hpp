@@ -1163,39 +1163,37 @@ __global__ __launch_bounds__(kBlock) void k_step_finish(StepArgs<R> a, const int
 }
 
 // Faces between two tiles: the lower cell's sum (hi) plus the upper cell's
-// (lo).  Entries are numbered i-seams, then j-seams, then k-seams.
+// (lo).  Entries are numbered i-seams, then j-seams, then k-seams; there are
+// fewer than half as many as cells, so 32-bit indices (and divisions) hold.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_face_seams(int IM, int JM, int KM, int64_t ni, int64_t nj, int64_t nk,
+__global__ __launch_bounds__(kBlock) void k_face_seams(int IM, int JM, int KM, int32_t ni, int32_t nj, int32_t nk,
                                                        FaceOut<R> fo) {
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int d, m, i = 0, j = 0, k = 0;
+  int32_t q = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  int d, i, j, k;
   if (q < ni) {
     d = 0;
-    j = (int)(q % JM);
-    const int64_t r = q / JM;
-    k = (int)(r % KM);
-    m = (int)(r / KM) + 1;
-    i = m * TX;
+    j = q % JM;
+    const int32_t r = q / JM;
+    k = r % KM;
+    i = (r / KM + 1) * TX;
   } else if ((q -= ni) < nj) {
     d = 1;
-    i = (int)(q % IM);
-    const int64_t r = q / IM;
-    k = (int)(r % KM);
-    m = (int)(r / KM) + 1;
-    j = m * TY;
+    i = q % IM;
+    const int32_t r = q / IM;
+    k = r % KM;
+    j = (r / KM + 1) * TY;
   } else if ((q -= nj) < nk) {
     d = 2;
-    i = (int)(q % IM);
-    const int64_t r = q / IM;
-    j = (int)(r % JM);
-    m = (int)(r / JM) + 1;
-    k = m * TZ;
+    i = q % IM;
+    const int32_t r = q / IM;
+    j = r % JM;
+    k = (r / JM + 1) * TZ;
   } else {
     return;
   }
   const int64_t f = face_index(d, i, j, k, IM, JM);
   R *face = fo.face[d];
-  const R *lo = fo.lo[d] + 4 * q, *hi = fo.hi[d] + 4 * q;
+  const R *lo = fo.lo[d] + 4 * (int64_t)q, *hi = fo.hi[d] + 4 * (int64_t)q;
   const auto vh = load4<R>(hi), vl = load4<R>(lo);
   store4<R>(face + 4 * f, vh.x + vl.x, vh.y + vl.y, vh.z + vl.z, vh.w + vl.w);
 }

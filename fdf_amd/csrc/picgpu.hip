@@ -834,7 +834,7 @@ static int do_rk2(picgpu_ctx *c, double dt) {
 // reads that index.  The swarm is compacted into key order every
 // kCompactEvery steps and at the end of picgpu_step.
 // ---------------------------------------------------------------------------
-static constexpr int kCompactEvery = 8;
+static constexpr int kCompactEvery = 16;
 
 static TileMap tile_map(const picgpu_ctx *c) {
   return TileMap{(c->IM + TX - 1) / TX, (c->JM + TY - 1) / TY, (c->KM + TZ - 1) / TZ};
@@ -946,7 +946,8 @@ static int launch_face_seams(picgpu_ctx *c, const void *fo_in) {
   return dispatch_real(c, [&](auto *tag) -> int {
     using R = std::remove_pointer_t<decltype(tag)>;
     const FaceOut<R> &fo = *static_cast<const FaceOut<R> *>(fo_in);
-    PG_LAUNCH(c, "face_seams", nblocks(ni + nj + nk), kBlock, k_face_seams<R>, c->IM, c->JM, c->KM, ni, nj, nk, fo);
+    PG_LAUNCH(c, "face_seams", nblocks(ni + nj + nk), kBlock, k_face_seams<R>, c->IM, c->JM, c->KM, (int32_t)ni,
+              (int32_t)nj, (int32_t)nk, fo);
     return PICGPU_OK;
   });
 }
