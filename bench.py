@@ -562,6 +562,8 @@ def main():
     ap.add_argument("--cpu-block", default="128,128,64",
                     help="cells of the CPU baseline's sub-block (timed with every core and with one)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-relocate", action="store_true",
+                    help="skip the config-4 relocation record that follows a config-5 run")
     ap.add_argument("--error-sample", type=int, default=1 << 22,
                     help="particles of the FP32-vs-FP64 one-step error sample (0: skip)")
     ap.add_argument("--fields", action="store_true",
@@ -667,7 +669,12 @@ def main():
         "config": {"workload": f"config{args.config}", "grid_cells": [d["IM"], d["JM"], d["KM"]],
                    "particles": d["n"], "scale": args.scale, "advection": "RK2" if d["rk2"] else "Euler",
                    "deposition": d["deposit"], "dt": d["dt"], "variant": head["variant"],
-                   "eulerian_preparation_each_step": bool(args.fields)},
+                   "eulerian_preparation_each_step": bool(args.fields),
+                   # particles keep their cell from step to step, so every walk after the
+                   # first starts from the previous cell; the MLP guess serves the first
+                   # locate (and config 4's relocation of fresh positions)
+                   "mlp_guess_per_step": False,
+                   "mlp_guess_at": "initial locate"},
         "roofline": {k: head["roofline"][k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
         "roofline_kernel": head["roofline"]["kernel"],
         "measured_copy_GBs": copy_gbs,
@@ -684,7 +691,38 @@ def main():
             log("fp32 error vs fp64:", line["fp32_error_vs_fp64"])
         except Exception as e:
             log("precision error sample failed:", e)
+    if args.config == 5 and not args.no_relocate:
+        del d
+        torch.cuda.empty_cache()
+        try:
+            line["config4_relocate"] = relocate_record(args.scale)
+        except Exception as e:
+            log("config 4 relocation record failed:", e)
     print(json.dumps(line), flush=True)
+
+
+def relocate_record(scale: float, steps: int = 3) -> dict:
+    """The MLP's score on its own workload (config 4: the bent tube at
+    128x128x1024, 16M fresh uniform positions located each step, cells
+    discarded): FP32 locates/s and walk iterations with the guess and from
+    (0,0,0) without it."""
+    import torch
+
+    d = make_inputs(4, scale, "cuda:0")
+    log(f"config 4 relocation: {d['IM']}x{d['JM']}x{d['KM']} cells, {d['n']} particles")
+    with_guess = run_relocate(d, "fp32", True, steps, 1)
+    without = run_relocate(d, "fp32", False, 1, 1)
+    keep = ("locates_per_s", "ms_per_locate", "walk_iterations", "located", "lost")
+    rec = {"workload": "config4", "grid_cells": [d["IM"], d["JM"], d["KM"]], "particles": int(d["n"]),
+           "precision": "fp32", "steps_timed": {"guess": steps, "no_guess": 1},
+           "guess": {k: with_guess[k] for k in keep}, "no_guess": {k: without[k] for k in keep},
+           "mlp_train": with_guess.get("mlp_train", {}).get("train_s")}
+    rec["walk_iterations_saved_per_locate"] = (rec["no_guess"]["walk_iterations"]["mean"] -
+                                               rec["guess"]["walk_iterations"]["mean"])
+    rec["speedup_from_guess"] = rec["guess"]["locates_per_s"] / rec["no_guess"]["locates_per_s"]
+    del d
+    torch.cuda.empty_cache()
+    return rec
 
 
 def run_relocate(d: dict, precision: str, use_mlp: bool, steps: int, warmup: int) -> dict:

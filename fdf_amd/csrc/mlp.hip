@@ -1,7 +1,8 @@
 // mlp.hip -- the learned initial guess for the walking search (gfx950).
 //
-// A 3 -> 64 -> 64 -> 3 MLP with erf GELU (erff in training; in inference a
-// branch-free erf within 1.5e-7, gelu_as) maps normalised physical
+// A 3 -> 64 -> 64 -> 3 MLP with GELU in its sigmoid form z sigma(1.702 z)
+// (Hendrycks & Gimpel, "Gaussian Error Linear Units", 2016, section 2; the
+// same function in training, inference and the oracle) maps normalised physical
 // coordinates to the fractional logical position ((i+a1)/IM, (j+a2)/JM,
 // (k+a3)/KM).  Weights are BF16, products accumulate in FP32; layer 1 runs on
 // the VALU (K = 3), layers 2 and 3 and the large backward products on MFMA
@@ -42,9 +43,12 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// GELU as z sigma(k z), k = 1.702, and its derivative sigma + k z sigma (1 - sigma)
+constexpr float kGeluK = 1.702f;
+__device__ __forceinline__ float gelu(float z) { return z / (1.0f + expf(-kGeluK * z)); }
 __device__ __forceinline__ float gelu_d(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * expf(-0.5f * z * z) * 0.39894228040143268f;
+  const float s = 1.0f / (1.0f + expf(-kGeluK * z));
+  return s + kGeluK * z * s * (1.0f - s);
 }
 
 // C[32x32] += A[32xK] * B[Kx32]; A(m,k) = A[m*sAm + k*sAk], B(k,n) = B[k*sBk + n*sBn]
@@ -151,22 +155,13 @@ __device__ __forceinline__ float norm_axis(double x, double lo, double s) { retu
 // v_cvt_pk_bf16_f32 instead of f2bf's integer sequence.
 __device__ __forceinline__ short bf16_rne(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 
-// GELU with erf(u) = 1 - P(t) exp(-u^2), t = 1 / (1 + p u), u >= 0
-// (Abramowitz & Stegun 7.1.26, |error| <= 1.5e-7), written so that neither
-// tail cancels: with q = P(t) exp(-u^2) at |u|, 1 + erf(u) is 2 - q for
-// u >= 0 and q for u < 0.  The reciprocal and exp2 are the hardware
-// approximations (v_rcp_f32, v_exp_f32, about 1 ulp each).
-__device__ __forceinline__ float gelu_as(float z) {
-  const float u = z * 0.70710678118654752f;
-  const float au = fabsf(u);
-  const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(0.3275911f, au, 1.0f));
-  float P = __builtin_fmaf(1.061405429f, t, -1.453152027f);
-  P = __builtin_fmaf(P, t, 1.421413741f);
-  P = __builtin_fmaf(P, t, -0.284496736f);
-  P = __builtin_fmaf(P, t, 0.254829592f);
-  P = P * t;
-  const float q = P * __builtin_amdgcn_exp2f(-(au * au) * 1.44269504088896341f);
-  return 0.5f * z * (u >= 0.f ? 2.0f - q : q);
+// Inference GELU: z sigma(k z) as z / (1 + 2^(-k log2(e) z)) with the hardware
+// exp2 and reciprocal (v_exp_f32, v_rcp_f32, about 1 ulp each; their error
+// is far below the BF16 rounding that follows).  Very negative z gives
+// 2^(+large) = inf and z * 0 = -0, as the division form does.
+__device__ __forceinline__ float gelu_fast(float z) {
+  constexpr float k2 = 2.4554669f;  // 1.702 log2(e)
+  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-k2 * z));
 }
 
 // Inference of 32 samples per wavefront; the weights are staged once per
@@ -180,11 +175,9 @@ __device__ __forceinline__ float gelu_as(float z) {
 // summation order are those of tile_forward: layer 1 as the same FP32
 // expression, products over k in ascending blocks of 8, biases added after.
 //
-// Inference evaluates GELU with gelu_as instead of erff: the same
-// 0.5 z (1 + erf(z / sqrt 2)), with erf from Abramowitz & Stegun 7.1.26.
-// Its erf error is at most 1.5e-7 absolute, below the BF16 rounding that
-// follows, and it has no branches (erff takes one of two paths per lane, and
-// a wavefront runs both).
+// Inference evaluates GELU with gelu_fast (hardware exp2 and reciprocal)
+// instead of expf and a division: two transcendental instructions and three
+// others per activation, 128 activations per sample.
 constexpr int INFER_WAVES = 4;      // wavefronts per block
 constexpr int INFER_BLOCKS = 1536;  // 6 wavefronts per SIMD; they loop over 32-sample tiles
 
@@ -259,7 +252,7 @@ __global__ __launch_bounds__(64 * INFER_WAVES, 4) void k_mlp_infer(int64_t n, co
       for (int v = 0; v < 4; ++v) {
         const float4 w = w1b[8 * kk + 4 * h + v];
         const float z = __builtin_fmaf(w.x, x0, __builtin_fmaf(w.y, x1, w.z * x2)) + w.w;
-        a1f[kk][v] = bf16_rne(gelu_as(z));
+        a1f[kk][v] = bf16_rne(gelu_fast(z));
       }
     // layer 2 (MFMA): column tile t holds hidden units 32t..32t+31
     bf16x4 a2f[8];
@@ -269,7 +262,7 @@ __global__ __launch_bounds__(64 * INFER_WAVES, 4) void k_mlp_infer(int64_t n, co
 #pragma unroll
       for (int kk = 0; kk < 8; ++kk) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w2s[t][kk][l], a1f[kk], acc, 0, 0, 0);
 #pragma unroll
-      for (int r = 0; r < 16; ++r) a2f[4 * t + r / 4][r % 4] = bf16_rne(gelu_as(acc[r] + b2s[32 * t + 8 * (r / 4) + 4 * h + (r % 4)]));
+      for (int r = 0; r < 16; ++r) a2f[4 * t + r / 4][r % 4] = bf16_rne(gelu_fast(acc[r] + b2s[32 * t + 8 * (r / 4) + 4 * h + (r % 4)]));
     }
     // layer 3 (MFMA): rows 0..2 of the result (registers 0..2 of lanes 0..31)
     f32x16 acc3 = {};

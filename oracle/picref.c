@@ -614,10 +614,12 @@ static double bf16_round(double v) {
   memcpy(&f, &u, 4);
   return (double)f;
 }
-static double gelu(double z) { return 0.5 * z * (1.0 + erf(z * 0.70710678118654752440)); }
+/* GELU in its sigmoid form z sigma(1.702 z) (Hendrycks & Gimpel 2016, sec. 2)
+   and its derivative sigma + 1.702 z sigma (1 - sigma) */
+static double gelu(double z) { return z / (1.0 + exp(-1.702 * z)); }
 static double gelu_d(double z) {
-  return 0.5 * (1.0 + erf(z * 0.70710678118654752440)) +
-         z * exp(-0.5 * z * z) * 0.39894228040143267794;
+  const double s = 1.0 / (1.0 + exp(-1.702 * z));
+  return s + 1.702 * z * s * (1.0 - s);
 }
 
 int64_t ref_mlp_param_count(int H) { return (int64_t)H * 3 + H + H * H + H + 3 * H + 3; }

@@ -167,7 +167,7 @@ def test_guess_locate_grouped_by_start_cell_matches_plain_locate(precision):
 
     n = (1 << 20) + 4099
     pos = np.random.default_rng(21).random((n, 3))
-    res = {}
+    res, grouped = {}, {}
     for use in (False, True):
         with PicGpu(precision=precision) as g:
             g.upload_grid(32, 32, 32, W.uniform_grid(32, 32, 32))
@@ -175,8 +175,15 @@ def test_guess_locate_grouped_by_start_cell_matches_plain_locate(precision):
                 g.mlp_set_params(M.initial_params(13))  # an untrained guess: any start cell
                 g.mlp_enable(True)
             g.upload_particles(pos)
+            g.timing(True)
+            g.timing_select("key_histogram")  # the first kernel of the grouping index
             g.LocateAllParticlesInGrid()
+            g.synchronize()
+            grouped[use] = g.timing_query("key_histogram")[1]
             res[use] = g.download_particles()
+    # grouping ran exactly where it is meant to: FP64 with a guess
+    assert grouped[False] == 0
+    assert grouped[True] == (1 if precision == "fp64" else 0)
     a, b = res[False], res[True]
     oa, ob = np.argsort(a["pid"]), np.argsort(b["pid"])
     for k in ("cell", "weight", "status"):

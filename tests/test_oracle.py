@@ -387,7 +387,7 @@ def test_mlp_forward_by_hand(oracle):
     b2[3] = -0.5
     W3[1, 3] = 3.0
     b3[:] = [0.1, 0.2, 0.3]
-    gelu = lambda z: 0.5 * z * (1 + math.erf(z / math.sqrt(2)))
+    gelu = lambda z: z / (1 + math.exp(-1.702 * z))  # GELU's sigmoid form
     x = np.array([0.3, -0.1, 0.7])
     z1 = 0.3 + 0.2 + 0.35 + 0.25
     z2 = 2.0 * gelu(z1) - 0.5
