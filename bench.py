@@ -68,17 +68,18 @@ def algorithmic_bytes(kernel: str, n: int, g: dict, rsize: int, deposit_share: f
     return float("nan")
 
 
-def design_bytes(kernel: str, n: int, g: dict, rsize: int) -> float:
+def design_bytes(kernel: str, n: int, g: dict, rsize: int, deposit_share: float = 0.0) -> float:
     """Bytes the implementation itself moves per launch of the fused step:
-    particle fields, one face-plane record per cell (128 B FP32, 304 B
-    FP64), the staged corner velocities and the faces."""
+    particle fields (in: index entry, cell, position; out: position, cell,
+    slot), one face-plane record per cell (128 B FP32, 304 B FP64), the
+    staged corner velocities and, on the launches that deposit, the faces
+    and the seam halves between tiles (about half a face per cell)."""
     if not kernel.startswith("step_fused"):
         return float("nan")
     rec = 128 if rsize == 4 else 304
-    # in: index, cell, position; out: position, cell, slot; one plane record
-    # per cell, the staged corner velocities, the faces
+    seams = g["ncell"] * 0.5 * 4 * rsize
     return n * ((4 + 4 + 24) + (24 + 4 + 4)) + g["ncell"] * rec + g["nnode"] * 3 * rsize + \
-        g["nface"] * 4 * rsize
+        deposit_share * (g["nface"] * 4 * rsize + seams)
 
 
 def step_bytes(n: int, g: dict, rsize: int, faces: bool) -> float:
@@ -89,10 +90,12 @@ def step_bytes(n: int, g: dict, rsize: int, faces: bool) -> float:
     return n * 2 * S + g["nnode"] * (24 + c) + g["ncell"] * c + g["nnode"] * c + D
 
 
-def measured_traffic(workload: str, variant: str, kernel: str) -> float | None:
+def measured_traffic(workload: str, variant: str, kernel: str, later_share: float) -> float | None:
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summaries (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration),
-    when one matches this workload and variant; otherwise None."""
+    summaries (the L2's memory requests counted by size: 32, 64 and 128 B),
+    when one matches this workload and variant; otherwise None.  The
+    summaries give the first launch of a call and the later ones (which also
+    deposit) separately; later_share is the fraction of later launches."""
     import glob
 
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.txt"))):
@@ -104,9 +107,11 @@ def measured_traffic(workload: str, variant: str, kernel: str) -> float | None:
                     kv[k.strip()] = v.strip()
         if kv.get("workload") == workload and kv.get("variant") == variant and kv.get("kernel") == kernel:
             try:
-                return float(kv["traffic_bytes_per_launch"])
+                first = float(kv["traffic_bytes_first_launch"])
+                later = float(kv["traffic_bytes_later_launch"])
             except (KeyError, ValueError):
                 return None
+            return (1.0 - later_share) * first + later_share * later
     return None
 
 
@@ -396,8 +401,9 @@ def roofline_for(res: dict, d: dict, g: dict) -> dict:
     achieved = b / (kern[dom]["avg_ms"] * 1e-3) / 1e9
     return {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(d["workload_name"], res["variant"], dom),
-            "algorithmic_bytes_per_launch": b, "design_bytes_per_launch": design_bytes(dom, int(n_avg), g, rsize),
+            "traffic": measured_traffic(d["workload_name"], res["variant"], dom, share),
+            "algorithmic_bytes_per_launch": b,
+            "design_bytes_per_launch": design_bytes(dom, int(n_avg), g, rsize, share),
             "avg_launch_ms": kern[dom]["avg_ms"], "particles_per_launch": n_avg,
             "share_of_step_time": kern[dom]["total_ms"] / (1e3 * res["seconds"])}
 
