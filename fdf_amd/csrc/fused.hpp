@@ -543,12 +543,12 @@ constexpr int32_t kLocalSlot = 0x40000000;
 // Start states (weights and velocity) of the first DepCap<R> particles of a
 // tile are kept in LDS for the tile's face sums; those of later particles
 // go to the w and vel arrays.  The FP32 step kernel's registers hold it to
-// four workgroups per CU, which leaves LDS for 896 entries (a tile holds
+// four workgroups per CU, which leaves LDS for 864 entries (a tile holds
 // about 512 particles at two per cell); the FP64 figure keeps three
 // workgroups' LDS within the CU's 160 KiB.
 template <typename R>
 struct DepCap {
-  static constexpr int value = sizeof(R) == 4 ? 896 : 352;
+  static constexpr int value = sizeof(R) == 4 ? 864 : 352;
 };
 enum OutKind : int8_t {
   OUT_NONE = 0,   // deferred to k_step_deferred: nothing is written
@@ -744,6 +744,22 @@ __device__ __forceinline__ void tile_face_sums(const StepArgs<R> &a, int t, int3
 // positions in LDS); more go to k_step_deferred.
 constexpr int kSlowCap = 256;
 
+// Where the fast pass leaves a particle whose walk left its cell: the walk
+// in progress (predictor, or corrector after a predictor that stayed in the
+// cell) evaluated the particle's own cell once, certified, and stepped to
+// the neighbour (di, dj, dk) -- walk_iter_lin's step rule on the same
+// distances.  The general loop continues that walk from the neighbour,
+// with one iteration counted and the own cell in the cycle guard, so the
+// walk is the one it would have run from the start.
+constexpr int kHandCont = 0x80, kHandCorr = 0x40;
+template <typename R>
+__device__ __forceinline__ int hand_step(const R d[6]) {
+  const int dk = d[F_FRONT] < (R)0 ? 1 : (d[F_BACK] < (R)0 ? -1 : 0);
+  const int di = d[F_LEFT] < (R)0 ? -1 : (d[F_RIGHT] < (R)0 ? 1 : 0);
+  const int dj = d[F_BOTTOM] < (R)0 ? -1 : (d[F_TOP] < (R)0 ? 1 : 0);
+  return kHandCont | (di + 1) | ((dj + 1) << 2) | ((dk + 1) << 4);
+}
+
 // One workgroup per tile: stage the tile's corner velocities in LDS, then
 // step the tile's particles idx[kstart[t*TS] .. kstart[(t+1)*TS]) in two
 // passes.
@@ -793,6 +809,13 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
   __shared__ R sd[CAPD][6];
   __shared__ int32_t shist[kWalkBins];
   __shared__ int32_t squeue[kSlowCap];
+  // FP32: the hand-over of each queued particle (0: from its start) and its
+  // predictor velocity, for a corrector walk continued.  FP64 starts every
+  // queued particle afresh: its start states already take the LDS of three
+  // workgroups per CU.
+  constexpr bool HAND = !EXACT;
+  __shared__ int8_t shand[HAND ? kSlowCap : 1];
+  __shared__ R sus[HAND && RK2 ? kSlowCap : 1][3];
   __shared__ int32_t sqn;
   const int t = step_tile(blockIdx.x, a.tm);
   const int32_t s = kstart[(int64_t)t * TS], e = kstart[(int64_t)(t + 1) * TS];
@@ -851,6 +874,8 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
     PState<R> psn;
     if (pn >= 0) load_state(a, pn, psn, !recompute);
     bool fast = ps.st == ST_LOCATED && ps.c0 >= 0;
+    int hand = 0;               // how the general loop takes the particle over
+    R us[3] = {0, 0, 0};        // predictor velocity (RK2)
     if (fast) {
       const int32_t lc = ps.c0;
       int ci, cj, ck;
@@ -883,14 +908,22 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
 #pragma unroll
       for (int c = 0; c < 3; ++c) x1[c] = ps.x0[c] + (double)ps.u0[c] * a.dt;
       R d[6];
-      fast = dist(x1, d) && point_accepted<R>(d);
+      bool cert = dist(x1, d);
+      fast = cert && point_accepted<R>(d);
+      bool corr = !RK2;  // the walk in progress is the corrector (Euler: the only walk)
       if (RK2 && fast) {
-        R ws[3], us[3];
+        R ws[3];
         weights_from_distances<R>(d, a.clamp, ws);
         interp_at<R>(src, ci, cj, ck, ws, us);
 #pragma unroll
         for (int c = 0; c < 3; ++c) x1[c] = ps.x0[c] + ((double)ps.u0[c] + (double)us[c]) * h;
-        fast = dist(x1, d) && point_accepted<R>(d);
+        cert = dist(x1, d);
+        fast = cert && point_accepted<R>(d);
+        corr = true;
+      }
+      if constexpr (HAND) {
+        if (!fast) hand = cert ? (hand_step<R>(d) | (corr ? kHandCorr : 0))
+                               : -1;  // an uncertified face: the FP64 routine of k_step_deferred
       }
       if (fast) {
         R w1[3] = {0, 0, 0}, u1[3] = {0, 0, 0};
@@ -914,12 +947,19 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
       }
     }
     if (!fast) {
-      const int r = atomicAdd(&sqn, 1);
+      const int r = hand < 0 ? kSlowCap : atomicAdd(&sqn, 1);
       if (r < kSlowCap) {
         squeue[r] = q;
+        if constexpr (HAND) {
+          shand[r] = (int8_t)hand;
+          if constexpr (RK2) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sus[r][c] = us[c];
+          }
+        }
       } else {
-        // more than the queue holds: stepped by k_step_deferred, which
-        // reads the start velocity from vel
+        // an uncertified face, or more than the queue holds: stepped by
+        // k_step_deferred, which reads the start velocity from vel
         a.defer_list[atomicAdd(a.defer_count, 1)] = p;
         if (recompute) {
 #pragma unroll
@@ -979,12 +1019,31 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) xw[c] = ps.x0[c] + (double)ps.u0[c] * a.dt;
-    if (ps.st == ST_REMOVED)
+    const int hc = HAND ? (uint8_t)shand[HAND ? r : 0] : 0;
+    if (ps.st == ST_REMOVED) {
       ph = 4;
-    else if (RK2 && ps.st == ST_LOST)
+    } else if (RK2 && ps.st == ST_LOST) {
       ph = 3;
-    else
+    } else if (hc & kHandCont) {
+      // the walk the fast pass began: own cell evaluated, step to the neighbour
+      cyc.revisit(lc);
+      steps = 1;
+      const int di = (hc & 3) - 1, dj = ((hc >> 2) & 3) - 1, dk = ((hc >> 4) & 3) - 1;
+      ci += di;
+      cj += dj;
+      ck += dk;
+      lc += di + dj * a.g.IM + dk * (a.g.IM * a.g.JM);
+      if (RK2 && (hc & kHandCorr)) {
+        ph = 2;
+        stot = 1;  // the predictor walk: one iteration, in the own cell
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xw[c] = ps.x0[c] + ((double)ps.u0[c] + (double)sus[r][c]) * h;
+      } else {
+        ph = RK2 ? 1 : 2;
+      }
+    } else {
       ph = RK2 ? 1 : 2;
+    }
   };
   if (p >= 0) begin();
 
