@@ -813,6 +813,7 @@ static int do_rk2(picgpu_ctx *c, double dt) {
     PG_CHECK(r);
   }
   PG_CHECK(do_locate(c, 1));
+  const int64_t pred_iters = c->stats.walk_iterations, pred_fallbacks = c->stats.fp64_fallbacks;
   PG_CHECK(do_interp(c));
   if (n > 0) {
     int r = dispatch_real(c, [&](auto *tag) -> int {
@@ -824,6 +825,9 @@ static int do_rk2(picgpu_ctx *c, double dt) {
     PG_CHECK(r);
   }
   PG_CHECK(do_locate(c, 1));
+  // a step's walk statistics cover both of its walks, as the fused step's do
+  c->stats.walk_iterations += pred_iters;
+  c->stats.fp64_fallbacks += pred_fallbacks;
   return PICGPU_OK;
 }
 

@@ -108,9 +108,11 @@ typedef struct {
   int64_t n_particles;      /* current swarm size */
   int64_t located;          /* ACTIVE_AND_LOCATED after the last locate */
   int64_t lost;             /* LOST after the last locate */
-  int64_t walk_iterations;  /* sum of walk iterations in the last locate */
+  int64_t walk_iterations;  /* sum of walk iterations in the last locate, or in the last
+                               step of picgpu_step (both walks of an RK2 step) */
   int64_t walk_max;         /* max walk iterations of one particle in the last locate */
-  int64_t fp64_fallbacks;   /* FP32 mode: faces re-evaluated in FP64 in the last locate */
+  int64_t fp64_fallbacks;   /* FP32 mode: faces re-evaluated in FP64 in the last locate
+                               (or the last step of picgpu_step) */
   int64_t removed_lost;     /* by the last remove calls */
   int64_t removed_out_of_bounds;
   int64_t particle_steps;   /* particles processed, summed over picgpu_step steps */

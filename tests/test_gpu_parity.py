@@ -435,7 +435,8 @@ def test_timing_select(swarm_cls):
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_long_fused_run_matches_staged(swarm_cls, precision):
     """20 steps in one call (the fused path compacts the swarm mid-call) give
-    the same particles as the stage-by-stage path."""
+    the same particles, and the same walk iterations, as the stage-by-stage
+    path."""
     _say("long_fused_vs_staged", precision)
     d = W.config(5, scale=0.02)
     d["pos"] = d["pos"][:60000]
@@ -454,6 +455,9 @@ def test_long_fused_run_matches_staged(swarm_cls, precision):
         assert np.array_equal(a[k], b[k]), k
     assert np.array_equal(fa, fb)
     assert sa["particle_steps"] == sb["particle_steps"]
+    # the fused walks (some continued from where the straight-line pass left
+    # them) take as many iterations as the stage kernels' walks
+    assert sa["walk_iterations"] == sb["walk_iterations"]
     assert len(a["pid"]) < 60000  # some particles left the domain
 
 
