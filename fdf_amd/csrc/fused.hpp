@@ -809,13 +809,13 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
   __shared__ R sd[CAPD][6];
   __shared__ int32_t shist[kWalkBins];
   __shared__ int32_t squeue[kSlowCap];
-  // FP32: the hand-over of each queued particle (0: from its start) and its
-  // predictor velocity, for a corrector walk continued.  FP64 starts every
-  // queued particle afresh: its start states already take the LDS of three
-  // workgroups per CU.
-  constexpr bool HAND = !EXACT;
-  __shared__ int8_t shand[HAND ? kSlowCap : 1];
-  __shared__ R sus[HAND && RK2 ? kSlowCap : 1][3];
+  // The hand-over of each queued particle (0: from its start) and, in FP32,
+  // its predictor velocity, for a corrector walk continued.  FP64 continues
+  // predictor walks only (RK2; Euler's one walk is a corrector walk): its
+  // start states already take most of three workgroups' LDS.
+  constexpr bool HAND_CORR = !EXACT || !RK2;
+  __shared__ int8_t shand[kSlowCap];
+  __shared__ R sus[!EXACT && RK2 ? kSlowCap : 1][3];
   __shared__ int32_t sqn;
   const int t = step_tile(blockIdx.x, a.tm);
   const int32_t s = kstart[(int64_t)t * TS], e = kstart[(int64_t)(t + 1) * TS];
@@ -921,9 +921,11 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
         fast = cert && point_accepted<R>(d);
         corr = true;
       }
-      if constexpr (HAND) {
-        if (!fast) hand = cert ? (hand_step<R>(d) | (corr ? kHandCorr : 0))
-                               : -1;  // an uncertified face: the FP64 routine of k_step_deferred
+      if (!fast) {
+        if (!cert)
+          hand = EXACT ? 0 : -1;  // FP32: an uncertified face, for the FP64 routine of k_step_deferred
+        else if (HAND_CORR || !corr)
+          hand = hand_step<R>(d) | (corr ? kHandCorr : 0);
       }
       if (fast) {
         R w1[3] = {0, 0, 0}, u1[3] = {0, 0, 0};
@@ -950,12 +952,10 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
       const int r = hand < 0 ? kSlowCap : atomicAdd(&sqn, 1);
       if (r < kSlowCap) {
         squeue[r] = q;
-        if constexpr (HAND) {
-          shand[r] = (int8_t)hand;
-          if constexpr (RK2) {
+        shand[r] = (int8_t)hand;
+        if constexpr (!EXACT && RK2) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) sus[r][c] = us[c];
-          }
+          for (int c = 0; c < 3; ++c) sus[r][c] = us[c];
         }
       } else {
         // an uncertified face, or more than the queue holds: stepped by
@@ -1019,7 +1019,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) xw[c] = ps.x0[c] + (double)ps.u0[c] * a.dt;
-    const int hc = HAND ? (uint8_t)shand[HAND ? r : 0] : 0;
+    const int hc = (uint8_t)shand[r];
     if (ps.st == ST_REMOVED) {
       ph = 4;
     } else if (RK2 && ps.st == ST_LOST) {
@@ -1033,11 +1033,11 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
       cj += dj;
       ck += dk;
       lc += di + dj * a.g.IM + dk * (a.g.IM * a.g.JM);
-      if (RK2 && (hc & kHandCorr)) {
+      if (!EXACT && RK2 && (hc & kHandCorr)) {
         ph = 2;
         stot = 1;  // the predictor walk: one iteration, in the own cell
 #pragma unroll
-        for (int c = 0; c < 3; ++c) xw[c] = ps.x0[c] + ((double)ps.u0[c] + (double)sus[r][c]) * h;
+        for (int c = 0; c < 3; ++c) xw[c] = ps.x0[c] + ((double)ps.u0[c] + (double)sus[!EXACT && RK2 ? r : 0][c]) * h;
       } else {
         ph = RK2 ? 1 : 2;
       }
