@@ -513,26 +513,28 @@ __device__ __forceinline__ void step_particle(const StepArgs<R> &a, const Src &s
   }
 }
 
-// one set of atomics per wavefront; every lane of the wave must call it
+// one set of atomics per wavefront; every lane of the wave must call it.  A
+// lane's particle counts fit 32 bits, so the wave sums them in 32 bits (one
+// shuffle per step instead of two); walk iterations keep 64.
 __device__ __forceinline__ void flush_tally(StepTally &t, unsigned long long *counters) {
-  unsigned long long v_loc = t.located, v_lost = t.lost, v_oob = t.oob, v_it = t.iters, v_fb = t.fb;
-  int v_err = t.err;
+  unsigned v_loc = (unsigned)t.located, v_lost = (unsigned)t.lost, v_oob = (unsigned)t.oob, v_fb = (unsigned)t.fb;
+  unsigned long long v_it = t.iters;
+  const bool any_err = __any(t.err != 0);
   for (int off = 32; off > 0; off >>= 1) {
     v_loc += __shfl_xor(v_loc, off);
     v_lost += __shfl_xor(v_lost, off);
     v_oob += __shfl_xor(v_oob, off);
     v_it += __shfl_xor(v_it, off);
     v_fb += __shfl_xor(v_fb, off);
-    v_err |= __shfl_xor(v_err, off);
   }
   if ((threadIdx.x & 63) == 0) {
     counters = counter_stripe(counters);
-    if (v_loc) atomicAdd(&counters[C_LOCATED], v_loc);
-    if (v_lost) atomicAdd(&counters[C_REMOVED_LOST], v_lost);
-    if (v_oob) atomicAdd(&counters[C_REMOVED_OOB], v_oob);
+    if (v_loc) atomicAdd(&counters[C_LOCATED], (unsigned long long)v_loc);
+    if (v_lost) atomicAdd(&counters[C_REMOVED_LOST], (unsigned long long)v_lost);
+    if (v_oob) atomicAdd(&counters[C_REMOVED_OOB], (unsigned long long)v_oob);
     if (v_it) atomicAdd(&counters[C_WALK_ITERS], v_it);
-    if (v_fb) atomicAdd(&counters[C_FALLBACKS], v_fb);
-    if (v_err) atomicOr(&counters[C_ERROR], 1ull);
+    if (v_fb) atomicAdd(&counters[C_FALLBACKS], (unsigned long long)v_fb);
+    if (any_err) atomicOr(&counters[C_ERROR], 1ull);
   }
 }
 
