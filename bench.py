@@ -160,7 +160,7 @@ def make_inputs(cfg: int, scale: float, device: str):
     return d
 
 
-def run_variant(name: str, d: dict, steps: int, warmup: int, mlp: dict | None):
+def run_variant(name: str, d: dict, steps: int, warmup: int, mlp: dict | None, call_steps: int = 0):
     import torch
 
     from fdf_amd.swarm import PicGpu
@@ -222,12 +222,17 @@ def run_variant(name: str, d: dict, steps: int, warmup: int, mlp: dict | None):
     ps0 = g.stats()["particle_steps"]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    g.AdvanceSimulation(d["dt"], steps, **flags)
+    if call_steps > 0:  # the timed steps as calls of call_steps steps each
+        for s0 in range(0, steps, call_steps):
+            g.AdvanceSimulation(d["dt"], min(call_steps, steps - s0), **flags)
+    else:
+        g.AdvanceSimulation(d["dt"], steps, **flags)
     g.synchronize()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     st = g.stats()
     res["seconds"] = t1 - t0
+    res["steps_per_call"] = call_steps if call_steps > 0 else steps
     res["particle_steps"] = st["particle_steps"] - ps0
     res["value"] = res["particle_steps"] / res["seconds"]
     res["ms_per_step"] = 1e3 * res["seconds"] / max(1, steps)
@@ -396,7 +401,9 @@ def roofline_for(res: dict, d: dict, g: dict) -> dict:
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     n_avg = res["particle_steps"] / max(1, res["steps"])
     faces = "faces" in d["deposit"] and "ucont" not in d  # per-step field preparation: a pass of its own
-    share = (res["steps"] - 1) / res["steps"] if faces and res["steps"] > 0 else 0.0
+    # every launch but the first of a call deposits the state it starts from
+    spc = min(res.get("steps_per_call", res["steps"]), res["steps"])
+    share = (spc - 1) / spc if faces and spc > 0 else 0.0
     b = algorithmic_bytes(dom, int(n_avg), g, rsize, share)
     achieved = b / (kern[dom]["avg_ms"] * 1e-3) / 1e9
     return {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -568,6 +575,8 @@ def main():
     ap.add_argument("--cpu-block", default="128,128,64",
                     help="cells of the CPU baseline's sub-block (timed with every core and with one)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--call-steps", type=int, default=0,
+                    help="time the steps as picgpu_step calls of this many steps each (0: one call)")
     ap.add_argument("--no-relocate", action="store_true",
                     help="skip the config-4 relocation record that follows a config-5 run")
     ap.add_argument("--error-sample", type=int, default=1 << 22,
@@ -625,7 +634,7 @@ def main():
 
     results = {}
     for v in [s.strip() for s in args.variants.split(",") if s.strip()]:
-        r = run_variant(v, d, steps, args.warmup, mlp)
+        r = run_variant(v, d, steps, args.warmup, mlp, args.call_steps)
         r["steps"] = steps
         r["roofline"] = roofline_for(r, d, g)
         rs = 8 if r["precision"] == "fp64" else 4
@@ -680,6 +689,7 @@ def main():
                    # first starts from the previous cell; the MLP guess serves the first
                    # locate (and config 4's relocation of fresh positions)
                    "mlp_guess_per_step": False,
+                   "steps_per_call": args.call_steps if args.call_steps > 0 else steps,
                    "mlp_guess_at": "initial locate"},
         "roofline": {k: head["roofline"][k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
         "roofline_kernel": head["roofline"]["kernel"],
