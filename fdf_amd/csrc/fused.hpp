@@ -770,16 +770,22 @@ __device__ __forceinline__ int hand_step(const R d[6]) {
 // whose walks each end in the cell they start from -- the predictor and the
 // corrector both accept the particle's own cell at the first evaluation,
 // every face certified in FP32 mode -- is stepped in straight-line code; one
-// plane record serves the start state, the predictor and the corrector.
+// plane record serves the start state, the predictor and the corrector.  A
+// walk that leaves the cell takes its step to the neighbour there too (one
+// step per particle step; in FP32 the neighbour's record replaces the own
+// one in the same registers), so that a particle that moves by one cell is
+// finished with the others of its wavefront.
 // A lane's next particle's state (and the index entry of the one after) is
 // requested before the current particle is stepped, and the first ones with
 // the staging loads.  Results are written as they come: the lanes of a
 // wavefront finish fast-pass particles together, at consecutive index
-// positions.  With WU = false the step ends there: the next step recomputes
-// the weights and velocity at the new position.  Every other particle (a walk
-// that leaves the cell, an uncertified face, a degenerate cell, a particle
-// that is not located at the start of the step) is queued for the general
-// loop.
+// positions, so that their lines are written whole (a result written later,
+// into a line already written back, costs a 32- or 64-byte write of its
+// own).  With WU = false the step ends there: the next step recomputes the
+// weights and velocity at the new position.  Every other particle (a walk
+// that needs a second step, an uncertified face, a degenerate cell, a
+// particle that is not located at the start of the step) is queued for the
+// general loop.
 //
 // General loop (lanes take the queued particles): all walk iterations of a
 // lane, across its particles, are one loop (predictor walk, corrector walk,
@@ -799,10 +805,14 @@ __device__ __forceinline__ int hand_step(const R d[6]) {
 // deposition costs no second pass over the particles.
 // WU = true: weights and velocities are written and the start velocity is
 // read (the field may change between steps); deposition is a kernel of its own.
+//
+// The FP32 kernel is held to 128 VGPRs (four wavefronts per SIMD, which its
+// LDS also allows).  The FP64 one is left to the compiler: held to three
+// wavefronts, the occupancy it has anyway, its schedule came out 14 % slower.
 template <typename R, bool EXACT, bool RK2, bool WU>
-__global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int32_t *__restrict__ idx,
-                                                       const int32_t *__restrict__ kstart,
-                                                       int32_t *__restrict__ kbase) {
+__global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 4 : 1) void k_step_tiled(
+    StepArgs<R> a, const int32_t *__restrict__ idx, const int32_t *__restrict__ kstart,
+    int32_t *__restrict__ kbase) {
   static_assert(TS == kBlock, "one thread per key of the tile");
   constexpr int CAPD = WU ? 1 : DepCap<R>::value;
   __shared__ R sk[BN * 3];
@@ -869,6 +879,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
   };
 
   // ---- fast pass ----
+  int32_t gp = -1, gv = 0;  // a slot taken in the global histogram, stored at the next one (or the end)
   while (p >= 0) {
     // the lane's next particle but one, and the next particle's state
     const int32_t q2 = q + 2 * B;
@@ -879,11 +890,12 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
     int hand = 0;               // how the general loop takes the particle over
     R us[3] = {0, 0, 0};        // predictor velocity (RK2)
     if (fast) {
-      const int32_t lc = ps.c0;
+      int32_t lc = ps.c0;  // the cell the walk in progress evaluates
       int ci, cj, ck;
       unpack_cell(lc, cell_idx(a.g), ci, cj, ck);
-      // FP32: the record, read once into registers, serves every evaluation;
-      // FP64: its 304 bytes are read from global memory where they are used
+      // FP32: the record, read once into registers, serves every evaluation
+      // in its cell; FP64: its 304 bytes are read from global memory where
+      // they are used
       CellPlanes32 rec32;
       if constexpr (!EXACT) rec32 = src.planes[lc];
       auto dist = [&](const double x[3], R d[6]) -> bool {  // certified distances
@@ -913,21 +925,49 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
       bool cert = dist(x1, d);
       fast = cert && point_accepted<R>(d);
       bool corr = !RK2;  // the walk in progress is the corrector (Euler: the only walk)
+      // One step of the walk, taken here: the walk in progress evaluated its
+      // cell (certified, not accepted); it steps to the neighbour by
+      // walk_iter_lin's rule and evaluates it (iteration 2 of that walk), when
+      // the neighbour is in the grid and MAX_TRAVERSAL allows a second
+      // iteration.  One step per particle step: mv 1 after the predictor's,
+      // 2 after the corrector's (Euler's walk), hs the step from the cell left.
+      int mv = 0, hs = 0;
+      auto step_once = [&]() {
+        hs = hand_step<R>(d);
+        const int di = (hs & 3) - 1, dj = ((hs >> 2) & 3) - 1, dk = ((hs >> 4) & 3) - 1;
+        const int ni = ci + di, nj = cj + dj, nk = ck + dk;
+        if (a.max_traversal >= 2 && ni >= 0 && ni < a.g.IM && nj >= 0 && nj < a.g.JM && nk >= 0 && nk < a.g.KM) {
+          ci = ni;
+          cj = nj;
+          ck = nk;
+          lc += di + dj * a.g.IM + dk * (a.g.IM * a.g.JM);
+          if constexpr (!EXACT) rec32 = src.planes[lc];
+          cert = dist(x1, d);
+          fast = cert && point_accepted<R>(d);
+          mv = corr ? 2 : 1;
+        }
+      };
+      if (cert && !fast) step_once();
       if (RK2 && fast) {
         R ws[3];
         weights_from_distances<R>(d, a.clamp, ws);
         interp_at<R>(src, ci, cj, ck, ws, us);
 #pragma unroll
         for (int c = 0; c < 3; ++c) x1[c] = ps.x0[c] + ((double)ps.u0[c] + (double)us[c]) * h;
-        cert = dist(x1, d);
-        fast = cert && point_accepted<R>(d);
         corr = true;
+        cert = dist(x1, d);  // the corrector walk starts in the predictor's cell
+        fast = cert && point_accepted<R>(d);
+        if (cert && !fast && mv == 0) step_once();
       }
       if (!fast) {
         if (!cert)
           hand = EXACT ? 0 : -1;  // FP32: an uncertified face, for the FP64 routine of k_step_deferred
-        else if (HAND_CORR || !corr)
-          hand = hand_step<R>(d) | (corr ? kHandCorr : 0);
+        else if (mv == 0)
+          hand = HAND_CORR || !corr ? hand_step<R>(d) | (corr ? kHandCorr : 0) : 0;
+        else if (mv == 1)
+          hand = corr ? 0 : hs;  // predictor continued from the neighbour; a corrector that left it restarts
+        else
+          hand = HAND_CORR ? hs | kHandCorr : 0;  // corrector continued from the neighbour
       }
       if (fast) {
         R w1[3] = {0, 0, 0}, u1[3] = {0, 0, 0};
@@ -935,19 +975,28 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
           weights_from_distances<R>(d, a.clamp, w1);
           interp_at<R>(src, ci, cj, ck, w1, u1);
         }
-        const int it = RK2 ? 2 : 1;
+        const int it = (RK2 ? 2 : 1) + (mv != 0);
         tally.iters += it;
         if (a.walk_hist) atomicAdd(&shist[walk_bin(it)], 1);
         const bool in = x1[0] >= a.bx0 && x1[0] <= a.bx1 && x1[1] >= a.by0 && x1[1] <= a.by1 && x1[2] >= a.bz0 &&
                         x1[2] <= a.bz1;
         int32_t slot = -1;
+        bool slot_later = false;
         if (in) {
           tally.located++;
-          slot = atomicAdd(&scount[cell_key(a.tm, ci, cj, ck) - t * TS], 1) | kLocalSlot;
+          const int32_t key = cell_key(a.tm, ci, cj, ck);
+          if (key / TS == t) {
+            slot = atomicAdd(&scount[key - t * TS], 1) | kLocalSlot;
+          } else {
+            if (gp >= 0) a.slot[gp] = gv;
+            gp = p;
+            gv = atomicAdd(&a.count_next[key], 1);
+            slot_later = true;
+          }
         } else {
           tally.oob++;
         }
-        emit_global<R, WU>(a, p, in ? OUT_LOC : OUT_OOB, x1, lc, w1, u1, slot);
+        emit_global<R, WU>(a, p, in ? OUT_LOC : OUT_OOB, x1, lc, w1, u1, slot, !slot_later);
       }
     }
     if (!fast) {
@@ -974,6 +1023,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tiled(StepArgs<R> a, const int3
     pn = p2;
     ps = psn;
   }
+  if (gp >= 0) a.slot[gp] = gv;
   __syncthreads();
 
   // ---- general loop over the queued particles ----
