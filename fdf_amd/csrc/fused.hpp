@@ -547,10 +547,16 @@ constexpr int32_t kLocalSlot = 0x40000000;
 // go to the w and vel arrays.  The FP32 step kernel's registers hold it to
 // four workgroups per CU, which leaves LDS for 864 entries (a tile holds
 // about 512 particles at two per cell); the FP64 figure keeps three
-// workgroups' LDS within the CU's 160 KiB.
+// workgroups' LDS within the CU's 160 KiB.  The face sums' exchange (12
+// values per cell) reuses the same LDS once the start states are summed.
 template <typename R>
 struct DepCap {
-  static constexpr int value = sizeof(R) == 4 ? 864 : 352;
+  static constexpr int value = sizeof(R) == 4 ? 864 : 608;
+};
+template <typename R, bool WU>
+struct StageLen {
+  static constexpr int dep = WU ? 1 : DepCap<R>::value;
+  static constexpr int value = WU ? 6 : (6 * dep > 12 * TS ? 6 * dep : 12 * TS);
 };
 enum OutKind : int8_t {
   OUT_NONE = 0,   // deferred to k_step_deferred: nothing is written
@@ -661,10 +667,11 @@ __device__ __forceinline__ void start_state(const StepArgs<R> &a, const Src &src
 // the LDS stage sd below DepCap, from w and vel beyond), once with weight
 // 1 - a (the cell's lower faces) and once with weight a (its upper faces),
 // per direction; a face is its lower cell's sum plus its upper cell's.  The
-// lower neighbour's sums come through LDS (xch) when it is in the tile; a
-// face between two tiles gets its halves in the seam arrays (FaceOut).
-// Every thread of the workgroup calls it, after a barrier that follows the
-// last write of a start state.
+// lower neighbour's sums come through LDS (xch: 12 rows of TS, in the LDS of
+// sd, which it overwrites) when it is in the tile; a face between two tiles
+// gets its halves in the seam arrays (FaceOut).  Every thread of the
+// workgroup calls it, after a barrier that follows the last write of a start
+// state.
 template <typename R, int CAPD>
 __device__ __forceinline__ void tile_face_sums(const StepArgs<R> &a, int t, int32_t s,
                                                const int32_t *__restrict__ idx, const int32_t *__restrict__ kstart,
@@ -712,18 +719,21 @@ __device__ __forceinline__ void tile_face_sums(const StepArgs<R> &a, int t, int3
   }
   const int g3[3] = {i, j, k}, l3[3] = {li, lj, lk}, dims[3] = {IM, JM, KM};
   constexpr int T3[3] = {TX, TY, TZ}, STR[3] = {1, TX, TX * TY};
+  __syncthreads();  // every thread's reads of sd are over: xch takes its place
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) xch[4 * d + v][l] = hh[d][v].value();
+  __syncthreads();
+  if (!own) return;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    __syncthreads();  // xch is free (and, the first time, the caller's LDS use is over)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) xch[v][l] = hh[d][v].value();
-    __syncthreads();
-    if (!own) continue;
     R *face = a.fo.face[d];
     const int64_t fl = face_index(d, i, j, k, IM, JM);
     if (l3[d] > 0) {  // lower neighbour in the tile
-      store4<R>(face + 4 * fl, xch[0][l - STR[d]] + hl[d][0].value(), xch[1][l - STR[d]] + hl[d][1].value(),
-                xch[2][l - STR[d]] + hl[d][2].value(), xch[3][l - STR[d]] + hl[d][3].value());
+      const int m = l - STR[d];
+      store4<R>(face + 4 * fl, xch[4 * d][m] + hl[d][0].value(), xch[4 * d + 1][m] + hl[d][1].value(),
+                xch[4 * d + 2][m] + hl[d][2].value(), xch[4 * d + 3][m] + hl[d][3].value());
     } else if (g3[d] == 0) {  // the domain's first face: no lower cell
       store4<R>(face + 4 * fl, (R)0 + hl[d][0].value(), (R)0 + hl[d][1].value(), (R)0 + hl[d][2].value(),
                 (R)0 + hl[d][3].value());
@@ -817,8 +827,10 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 4 : 1) void k_step_tiled(
   constexpr int CAPD = WU ? 1 : DepCap<R>::value;
   __shared__ R sk[BN * 3];
   __shared__ int32_t scount[TS];
-  __shared__ R xch[4][TS];  // the face-sum exchange
-  __shared__ R sd[CAPD][6];
+  // the start states to deposit (sd), then the face-sum exchange (xch)
+  __shared__ R sdx[StageLen<R, WU>::value];
+  R(*const sd)[6] = reinterpret_cast<R(*)[6]>(sdx);
+  R(*const xch)[TS] = reinterpret_cast<R(*)[TS]>(sdx);
   __shared__ int32_t shist[kWalkBins];
   __shared__ int32_t squeue[kSlowCap];
   // The hand-over of each queued particle (0: from its start) and, in FP32,
@@ -1208,8 +1220,9 @@ __global__ __launch_bounds__(kBlock) void k_step_finish(StepArgs<R> a, const int
                                                         const int32_t *__restrict__ kstart) {
   constexpr int CAPD = DepCap<R>::value;
   __shared__ R sk[BN * 3];
-  __shared__ R xch[4][TS];
-  __shared__ R sd[CAPD][6];
+  __shared__ R sdx[StageLen<R, false>::value];
+  R(*const sd)[6] = reinterpret_cast<R(*)[6]>(sdx);
+  R(*const xch)[TS] = reinterpret_cast<R(*)[TS]>(sdx);
   const int t = step_tile(blockIdx.x, a.tm);
   const int32_t s = kstart[(int64_t)t * TS], e = kstart[(int64_t)(t + 1) * TS];
   if (s == e && !a.deposit) return;
