@@ -1,8 +1,8 @@
 // mlp.hip -- the learned initial guess for the walking search (gfx950).
 //
-// A 3 -> 64 -> 64 -> 3 MLP with GELU in its sigmoid form z sigma(1.702 z)
-// (Hendrycks & Gimpel, "Gaussian Error Linear Units", 2016, section 2; the
-// same function in training, inference and the oracle) maps normalised physical
+// A 3 -> 64 -> 64 -> 3 MLP with the hard-swish activation
+// z clamp(z/6 + 1/2, 0, 1) (Howard et al., "Searching for MobileNetV3", 2019,
+// section 5.2; the same function in training, inference and the oracle) maps normalised physical
 // coordinates to the fractional logical position ((i+a1)/IM, (j+a2)/JM,
 // (k+a3)/KM).  Weights are BF16, products accumulate in FP32; layer 1 runs on
 // the VALU (K = 3), layers 2 and 3 and the large backward products on MFMA
@@ -13,9 +13,11 @@
 // the walk itself is unchanged, so ownership stays the reference's.
 //
 // Rounding rule (the CPU oracle states the same): weights W1, W2, W3 and
-// the matmul inputs a1 = gelu(z1), a2 = gelu(z2) and the backward g2 are
+// the matmul inputs a1 = act(z1), a2 = act(z2) and the backward g2 are
 // rounded to BF16 (nearest even); x, biases, z, losses and every
-// accumulation are FP32.
+// accumulation are FP32.  Biases are the first term of their sums: layer 1
+// is fma(w0, x0, fma(w1, x1, fma(w2, x2, b))), and the products of layers 2
+// and 3 accumulate onto the bias.
 #include <math.h>
 #include <string.h>
 
@@ -43,12 +45,18 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-// GELU as z sigma(k z), k = 1.702, and its derivative sigma + k z sigma (1 - sigma)
-constexpr float kGeluK = 1.702f;
-__device__ __forceinline__ float gelu(float z) { return z / (1.0f + expf(-kGeluK * z)); }
-__device__ __forceinline__ float gelu_d(float z) {
-  const float s = 1.0f / (1.0f + expf(-kGeluK * z));
-  return s + kGeluK * z * s * (1.0f - s);
+// Hard-swish z t(z), gate t = clamp(fma(z, c, 1/2), 0, 1) with c = fl32(1/6):
+// one fused multiply-add with the VOP3 clamp and one multiply, no
+// transcendental.  Derivative t + c z where the gate is strictly inside
+// (0, 1), t (0 or 1) where it is clamped.
+constexpr float kActC = 1.0f / 6.0f;
+__device__ __forceinline__ float act_gate(float z) {
+  return __builtin_amdgcn_fmed3f(__builtin_fmaf(z, kActC, 0.5f), 0.0f, 1.0f);
+}
+__device__ __forceinline__ float act(float z) { return z * act_gate(z); }
+__device__ __forceinline__ float act_d(float z) {
+  const float u = __builtin_fmaf(z, kActC, 0.5f);
+  return (u > 0.0f && u < 1.0f) ? __builtin_fmaf(kActC, z, u) : act_gate(z);
 }
 
 // C[32x32] += A[32xK] * B[Kx32]; A(m,k) = A[m*sAm + k*sAk], B(k,n) = B[k*sBk + n*sBn]
@@ -99,32 +107,37 @@ __device__ void tile_forward(const MlpWeights &Wt, TileLds &L) {
     const float w0 = bf2f(Wt.wb[OFF_W1 + 3 * q]), w1 = bf2f(Wt.wb[OFF_W1 + 3 * q + 1]),
                 w2 = bf2f(Wt.wb[OFF_W1 + 3 * q + 2]), b = Wt.p[OFF_B1 + q];
     for (int m = 0; m < TILE; ++m) {
-      float z = __builtin_fmaf(w0, L.x[m][0], __builtin_fmaf(w1, L.x[m][1], w2 * L.x[m][2])) + b;
+      float z = __builtin_fmaf(w0, L.x[m][0], __builtin_fmaf(w1, L.x[m][1], __builtin_fmaf(w2, L.x[m][2], b)));
       L.z1[m][q] = z;
-      L.a1[m][q] = f2bf(gelu(z));
+      L.a1[m][q] = f2bf(act(z));
     }
   }
   __syncthreads();
   // layer 2 (MFMA): Z2 = A1 (32x64) * W2^T, two 32-column tiles
   for (int t = 0; t < 2; ++t) {
-    f32x16 acc = {};
-    mm32(&L.a1[0][0], H, 1, Wt.wb + OFF_W2 + 32 * t * H, 1, H, H, acc);
     const int n = 32 * t + res_col();
     const float b = Wt.p[OFF_B2 + n];
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = b;  // the bias is the first term of each sum
+    mm32(&L.a1[0][0], H, 1, Wt.wb + OFF_W2 + 32 * t * H, 1, H, H, acc);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int m = res_row(r);
-      const float z = acc[r] + b;
+      const float z = acc[r];
       L.z2[m][n] = z;
-      L.a2[m][n] = f2bf(gelu(z));
+      L.a2[m][n] = f2bf(act(z));
     }
   }
   __syncthreads();
   // layer 3 (MFMA): Y = A2 (32x64) * W3^T, columns 0..2 of one tile
   {
-    f32x16 acc = {};
     const int l3 = res_col();
     const int h = l >> 5;
+    const float b3 = l3 < 3 ? Wt.p[OFF_B3 + l3] : 0.0f;
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = b3;
     for (int k0 = 0; k0 < H; k0 += 8) {
       bf16x4 a, b;
 #pragma unroll
@@ -136,9 +149,8 @@ __device__ void tile_forward(const MlpWeights &Wt, TileLds &L) {
       acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
     }
     if (l3 < 3) {
-      const float b = Wt.p[OFF_B3 + l3];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) L.y[res_row(r)][l3] = acc[r] + b;
+      for (int r = 0; r < 16; ++r) L.y[res_row(r)][l3] = acc[r];
     }
   }
   __syncthreads();
@@ -155,15 +167,6 @@ __device__ __forceinline__ float norm_axis(double x, double lo, double s) { retu
 // v_cvt_pk_bf16_f32 instead of f2bf's integer sequence.
 __device__ __forceinline__ short bf16_rne(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 
-// Inference GELU: z sigma(k z) as z / (1 + 2^(-k log2(e) z)) with the hardware
-// exp2 and reciprocal (v_exp_f32, v_rcp_f32, about 1 ulp each; their error
-// is far below the BF16 rounding that follows).  Very negative z gives
-// 2^(+large) = inf and z * 0 = -0, as the division form does.
-__device__ __forceinline__ float gelu_fast(float z) {
-  constexpr float k2 = 2.4554669f;  // 1.702 log2(e)
-  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-k2 * z));
-}
-
 // Inference of 32 samples per wavefront; the weights are staged once per
 // block in LDS, in the operand layout of the products.
 // Layers 2 and 3 run transposed (Z2^T = W2 A1^T, Y^T = W3 A2^T), so each
@@ -173,11 +176,9 @@ __device__ __forceinline__ float gelu_fast(float z) {
 // result register 4q + v of column tile t is hidden unit 32t + 8q + 4h + v,
 // which is B-operand slot (kk = 4t + q, v) of the next product.  Rounding and
 // summation order are those of tile_forward: layer 1 as the same FP32
-// expression, products over k in ascending blocks of 8, biases added after.
-//
-// Inference evaluates GELU with gelu_fast (hardware exp2 and reciprocal)
-// instead of expf and a division: two transcendental instructions and three
-// others per activation, 128 activations per sample.
+// expression, products over k in ascending blocks of 8 accumulated onto the
+// bias.  Per activation the VALU does the gate's clamped fma, a multiply and
+// half a BF16 pair conversion (plus layer 1's three fmas).
 constexpr int INFER_WAVES = 4;      // wavefronts per block
 constexpr int INFER_BLOCKS = 1536;  // 6 wavefronts per SIMD; they loop over 32-sample tiles
 
@@ -251,25 +252,30 @@ __global__ __launch_bounds__(64 * INFER_WAVES, 4) void k_mlp_infer(int64_t n, co
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const float4 w = w1b[8 * kk + 4 * h + v];
-        const float z = __builtin_fmaf(w.x, x0, __builtin_fmaf(w.y, x1, w.z * x2)) + w.w;
-        a1f[kk][v] = bf16_rne(gelu_fast(z));
+        const float z = __builtin_fmaf(w.x, x0, __builtin_fmaf(w.y, x1, __builtin_fmaf(w.z, x2, w.w)));
+        a1f[kk][v] = bf16_rne(act(z));
       }
     // layer 2 (MFMA): column tile t holds hidden units 32t..32t+31
     bf16x4 a2f[8];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      f32x16 acc = {};
+      f32x16 acc;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[r] = b2s[32 * t + 8 * (r / 4) + 4 * h + (r % 4)];
 #pragma unroll
       for (int kk = 0; kk < 8; ++kk) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w2s[t][kk][l], a1f[kk], acc, 0, 0, 0);
 #pragma unroll
-      for (int r = 0; r < 16; ++r) a2f[4 * t + r / 4][r % 4] = bf16_rne(gelu_fast(acc[r] + b2s[32 * t + 8 * (r / 4) + 4 * h + (r % 4)]));
+      for (int r = 0; r < 16; ++r) a2f[4 * t + r / 4][r % 4] = bf16_rne(act(acc[r]));
     }
     // layer 3 (MFMA): rows 0..2 of the result (registers 0..2 of lanes 0..31)
     f32x16 acc3 = {};
+    acc3[0] = b30;  // rows 0..2 of the result: the outputs, accumulated onto their biases
+    acc3[1] = b31;
+    acc3[2] = b32;
 #pragma unroll
     for (int kk = 0; kk < 8; ++kk) acc3 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w3s[kk][l], a2f[kk], acc3, 0, 0, 0);
     if (h == 0 && in) {
-      const float y0 = acc3[0] + b30, y1 = acc3[1] + b31, y2 = acc3[2] + b32;
+      const float y0 = acc3[0], y1 = acc3[1], y2 = acc3[2];
       if (GUESS) {
         int32_t g = -1;
         if (need) {
@@ -354,7 +360,7 @@ __global__ __launch_bounds__(MLP_THREADS) void k_mlp_fwd_bwd(int64_t n, const fl
       gw1 = __builtin_fmaf(L.g3[m][1], a2, gw1);
       gw2 = __builtin_fmaf(L.g3[m][2], a2, gw2);
       const float da2 = __builtin_fmaf(L.g3[m][0], w30, __builtin_fmaf(L.g3[m][1], w31, L.g3[m][2] * w32));
-      const float g2 = da2 * gelu_d(L.z2[m][nn]);
+      const float g2 = da2 * act_d(L.z2[m][nn]);
       gb2 += g2;
       L.g2[m][nn] = f2bf(g2);
     }
@@ -384,7 +390,7 @@ __global__ __launch_bounds__(MLP_THREADS) void k_mlp_fwd_bwd(int64_t n, const fl
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int m = res_row(r), q = 32 * tq + res_col();
-      L.g1[m][q] = acc[r] * gelu_d(L.z1[m][q]);
+      L.g1[m][q] = acc[r] * act_d(L.z1[m][q]);
     }
   }
   __syncthreads();
@@ -406,14 +412,28 @@ __global__ __launch_bounds__(MLP_THREADS) void k_mlp_fwd_bwd(int64_t n, const fl
   }
 }
 
-// grad[j] = sum over tiles (fixed order) of part[tile][j], j < NP + 1
-__global__ __launch_bounds__(256) void k_mlp_reduce(int ntiles, const float *__restrict__ part,
-                                                    float *__restrict__ grad) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j > NP) return;
+// grad[j] = sum over tiles of part[tile][j], j < NP + 1, in a fixed order:
+// wavefront g of a block sums the tiles b = g (mod RED_GROUPS) in ascending
+// order, then the RED_GROUPS partial sums are added in ascending g.  Each
+// block covers 64 consecutive j, one per lane, so every load is a coalesced
+// 256-byte row segment and RED_GROUPS loads are in flight per j.
+constexpr int RED_GROUPS = 8;
+__global__ __launch_bounds__(64 * RED_GROUPS) void k_mlp_reduce(int ntiles, const float *__restrict__ part,
+                                                                float *__restrict__ grad) {
+  __shared__ float ps[RED_GROUPS][64];
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + lane;
   float s = 0.f;
-  for (int b = 0; b < ntiles; ++b) s += part[(int64_t)b * (NP + 1) + j];
-  grad[j] = s;
+  if (j <= NP)
+    for (int b = grp; b < ntiles; b += RED_GROUPS) s += part[(int64_t)b * (NP + 1) + j];
+  ps[grp][lane] = s;
+  __syncthreads();
+  if (grp == 0 && j <= NP) {
+    float t = ps[0][lane];
+#pragma unroll
+    for (int q = 1; q < RED_GROUPS; ++q) t += ps[q][lane];
+    grad[j] = t;
+  }
 }
 
 // Adam on the FP32 master, then refresh the BF16 weight copy.
@@ -531,7 +551,7 @@ static int loss_grad_device(picgpu_ctx *c, int64_t n, const float *dx, const flo
   const float scale = 2.0f / (3.0f * (float)n);
   PG_LAUNCH(c, "mlp_fwd_bwd", tiles, MLP_THREADS, k_mlp_fwd_bwd, n, dx, dt, npairs, seed, iter, scale, weights(c),
             c->mlp_partial.as<float>());
-  PG_LAUNCH(c, "mlp_reduce", (NP + 1 + 255) / 256, 256, k_mlp_reduce, (int)tiles, c->mlp_partial.as<float>(),
+  PG_LAUNCH(c, "mlp_reduce", (NP + 1 + 63) / 64, 64 * RED_GROUPS, k_mlp_reduce, (int)tiles, c->mlp_partial.as<float>(),
             c->mlp_grad.as<float>());
   if (loss_host) {
     float sq = 0.f;

@@ -614,12 +614,18 @@ static double bf16_round(double v) {
   memcpy(&f, &u, 4);
   return (double)f;
 }
-/* GELU in its sigmoid form z sigma(1.702 z) (Hendrycks & Gimpel 2016, sec. 2)
-   and its derivative sigma + 1.702 z sigma (1 - sigma) */
-static double gelu(double z) { return z / (1.0 + exp(-1.702 * z)); }
-static double gelu_d(double z) {
-  const double s = 1.0 / (1.0 + exp(-1.702 * z));
-  return s + 1.702 * z * s * (1.0 - s);
+/* Hard-swish z t(z) with the gate t = clamp(c z + 1/2, 0, 1), c = the binary32
+   value of 1/6 (Howard et al. 2019, "Searching for MobileNetV3", sec. 5.2).
+   Derivative: t + c z where 0 < c z + 1/2 < 1, else t (0 or 1). */
+static const double act_c = (double)(1.0f / 6.0f);
+static double act_gate(double z) {
+  const double u = z * act_c + 0.5;
+  return u < 0.0 ? 0.0 : (u > 1.0 ? 1.0 : u);
+}
+static double act(double z) { return z * act_gate(z); }
+static double act_d(double z) {
+  const double u = z * act_c + 0.5;
+  return (u > 0.0 && u < 1.0) ? u + act_c * z : act_gate(z);
 }
 
 int64_t ref_mlp_param_count(int H) { return (int64_t)H * 3 + H + H * H + H + 3 * H + 3; }
@@ -641,24 +647,25 @@ static mlp_view_t mlp_view(int H, const double *pr) {
 static void mlp_fwd_one(int H, const double *Wr, mlp_view_t v, const double x[3], double *z1,
                         double *a1r, double *z2, double *a2r, double y[3], int rb) {
   (void)Wr;
+  /* each sum starts from its bias; layer 1 adds x2, x1, x0 in that order */
   for (int o = 0; o < H; ++o) {
-    double s = 0.0;
-    for (int q = 0; q < 3; ++q) s += v.W1[3 * o + q] * x[q];
-    z1[o] = s + v.b1[o];
-    double a = gelu(z1[o]);
+    double s = v.b1[o];
+    for (int q = 2; q >= 0; --q) s += v.W1[3 * o + q] * x[q];
+    z1[o] = s;
+    double a = act(z1[o]);
     a1r[o] = rb ? bf16_round(a) : a;
   }
   for (int o = 0; o < H; ++o) {
-    double s = 0.0;
+    double s = v.b2[o];
     for (int q = 0; q < H; ++q) s += v.W2[H * o + q] * a1r[q];
-    z2[o] = s + v.b2[o];
-    double a = gelu(z2[o]);
+    z2[o] = s;
+    double a = act(z2[o]);
     a2r[o] = rb ? bf16_round(a) : a;
   }
   for (int o = 0; o < 3; ++o) {
-    double s = 0.0;
+    double s = v.b3[o];
     for (int q = 0; q < H; ++q) s += v.W3[H * o + q] * a2r[q];
-    y[o] = s + v.b3[o];
+    y[o] = s;
   }
 }
 
@@ -718,7 +725,7 @@ double ref_mlp_loss_grad(int H, const double *params, int64_t n, const double *x
     for (int q = 0; q < H; ++q) {
       double s = 0.0;
       for (int o = 0; o < 3; ++o) s += g3[o] * v.W3[H * o + q];
-      g2[q] = s * gelu_d(z2[q]);
+      g2[q] = s * act_d(z2[q]);
       g2r[q] = rb ? bf16_round(g2[q]) : g2[q];
     }
     for (int o = 0; o < H; ++o) {
@@ -728,7 +735,7 @@ double ref_mlp_loss_grad(int H, const double *params, int64_t n, const double *x
     for (int q = 0; q < H; ++q) {
       double s = 0.0;
       for (int o = 0; o < H; ++o) s += g2r[o] * v.W2[H * o + q];
-      g1[q] = s * gelu_d(z1[q]);
+      g1[q] = s * act_d(z1[q]);
     }
     for (int o = 0; o < H; ++o) {
       for (int q = 0; q < 3; ++q) gW1[3 * o + q] += g1[o] * x[3 * p + q];

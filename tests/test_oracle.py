@@ -348,14 +348,18 @@ def test_rk2_uniform_flow_exact(oracle):
 
 
 # ------------------------------------------------------------------------- MLP
-def _mlp_params(seed, H=64):
+def _mlp_params(seed, H=64, w1=0.7):
     rng = np.random.default_rng(seed)
-    return np.concatenate([rng.normal(0, 0.7, 3 * H), rng.normal(0, 0.1, H), rng.normal(0, 1 / 8, H * H),
+    return np.concatenate([rng.normal(0, w1, 3 * H), rng.normal(0, 0.1, H), rng.normal(0, 1 / 8, H * H),
                            rng.normal(0, 0.1, H), rng.normal(0, 1 / 8, 3 * H), rng.normal(0, 0.1, 3)])
 
 
-def test_mlp_gradient_matches_finite_differences(oracle):
-    p = _mlp_params(1)
+@pytest.mark.parametrize("w1", [0.7, 4.0])
+def test_mlp_gradient_matches_finite_differences(oracle, w1):
+    """Central differences; with first-layer weights of scale 4 about half of
+    the first-layer gates are clamped, so both branches of the activation's
+    derivative are exercised."""
+    p = _mlp_params(1, w1=w1)
     rng = np.random.default_rng(2)
     x = rng.uniform(-1, 1, (64, 3))
     t = rng.uniform(0, 1, (64, 3))
@@ -373,7 +377,9 @@ def test_mlp_gradient_matches_finite_differences(oracle):
 
 
 def test_mlp_forward_by_hand(oracle):
-    """One hidden unit through each layer, written out."""
+    """One hidden unit through each layer, written out, with the hard-swish
+    gate inside (0, 1) on the way in and saturated at 1 and at 0 on two
+    more units."""
     H = 64
     p = np.zeros(oracle.lib.ref_mlp_param_count(H))
     W1, b1 = p[:3 * H].reshape(H, 3), p[3 * H:4 * H]
@@ -387,11 +393,16 @@ def test_mlp_forward_by_hand(oracle):
     b2[3] = -0.5
     W3[1, 3] = 3.0
     b3[:] = [0.1, 0.2, 0.3]
-    gelu = lambda z: z / (1 + math.exp(-1.702 * z))  # GELU's sigmoid form
+    b2[5] = 7.0    # z = 7: gate clamped at 1, act = 7
+    W3[0, 5] = 0.5
+    b2[6] = -7.0   # z = -7: gate clamped at 0, act = 0
+    W3[2, 6] = 4.0
+    c = float(np.float32(1.0) / np.float32(6.0))  # the binary32 value of 1/6
+    act = lambda z: z * min(max(c * z + 0.5, 0.0), 1.0)  # hard-swish
     x = np.array([0.3, -0.1, 0.7])
     z1 = 0.3 + 0.2 + 0.35 + 0.25
-    z2 = 2.0 * gelu(z1) - 0.5
+    z2 = 2.0 * act(z1) - 0.5
+    assert 0 < c * z1 + 0.5 < 1 and 0 < c * z2 + 0.5 < 1
     y = oracle.mlp_forward(p, x[None], round_bf16=False)[0]
-    g0 = gelu(0.0)
-    np.testing.assert_allclose(y, [0.1, 0.2 + 3.0 * gelu(z2), 0.3], rtol=1e-14)
-    assert g0 == 0.0
+    np.testing.assert_allclose(y, [0.1 + 0.5 * 7.0, 0.2 + 3.0 * act(z2), 0.3], rtol=1e-14)
+    assert act(0.0) == 0.0 and act(3.0 / c) == 3.0 / c and act(-3.0 / c) == 0.0
