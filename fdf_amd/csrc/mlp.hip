@@ -163,24 +163,41 @@ struct MlpNorm {
 };
 __device__ __forceinline__ float norm_axis(double x, double lo, double s) { return (float)((x - lo) * s - 1.0); }
 
-// FP32 -> BF16 bits, rounded to nearest even like f2bf; on gfx950 a single
-// v_cvt_pk_bf16_f32 instead of f2bf's integer sequence.
-__device__ __forceinline__ short bf16_rne(float f) { return __builtin_bit_cast(short, (__bf16)f); }
+// Inference of 32 samples per wavefront, every layer on MFMA, the weights in
+// registers.  The layers run transposed (Z1^T = W1 X^T, Z2^T = W2 A1^T,
+// Y^T = W3 A2^T): lane l works on sample n = l%32 with h = l/32, and the
+// result register r of a 32-unit column tile t is hidden unit
+// 32t + 8(r/4) + 4h + r%4 of that sample, so a layer's results are already
+// in B-operand registers of the next product.
+//  - Layer 1: v_mfma_f32_32x32x2_f32 (FP32 operands, K = 2): two products per
+//    column tile, k = (x0, x1) and (x2, 1), the 1 carrying the bias.
+//  - Layers 2 and 3: gfx950's v_mfma_f32_32x32x16_bf16 (K = 16), four
+//    products per column tile.  K block kb draws its B operand from the
+//    registers 8(kb%2)..8(kb%2)+7 of column tile kb/2 of the previous layer,
+//    so its k slot s stands for hidden unit infer_unit(kb, s); the weight
+//    fragments are gathered in that order once per wavefront.  Layer 2
+//    accumulates onto its bias, layer 3 onto its three.
+// Sums are the network's; their rounding order is the hardware's (layer 1
+// adds the bias with the products instead of first).  Per activation the
+// VALU does the gate's clamped fma, a multiply and half a BF16 pair
+// conversion.
+__host__ __device__ constexpr int infer_unit(int kb, int s) {
+  return 32 * (kb >> 1) + 8 * (2 * (kb & 1) + ((s & 7) >> 2)) + 4 * (s >> 3) + (s & 3);
+}
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+// the K blocks of a product cover each hidden unit exactly once
+constexpr bool infer_units_cover() {
+  unsigned long long seen = 0;
+  for (int kb = 0; kb < 4; ++kb)
+    for (int s = 0; s < 16; ++s) seen |= 1ull << infer_unit(kb, s);
+  return seen == ~0ull;
+}
+static_assert(infer_units_cover(), "K-block unit order");
+// FP32 -> BF16 conversions ((__bf16)f) round to nearest even like f2bf; on
+// gfx950 they are v_cvt_pk_bf16_f32, two values per instruction.
 
-// Inference of 32 samples per wavefront; the weights are staged once per
-// block in LDS, in the operand layout of the products.
-// Layers 2 and 3 run transposed (Z2^T = W2 A1^T, Y^T = W3 A2^T), so each
-// layer's MFMA result is already in the operand layout of the next one:
-// lane l works on sample l%32 and, with h = l/32, holds hidden units
-// 8kk + 4h + v (kk = 0..7, v = 0..3) as the B operand of a product; the
-// result register 4q + v of column tile t is hidden unit 32t + 8q + 4h + v,
-// which is B-operand slot (kk = 4t + q, v) of the next product.  Rounding and
-// summation order are those of tile_forward: layer 1 as the same FP32
-// expression, products over k in ascending blocks of 8 accumulated onto the
-// bias.  Per activation the VALU does the gate's clamped fma, a multiply and
-// half a BF16 pair conversion (plus layer 1's three fmas).
 constexpr int INFER_WAVES = 4;      // wavefronts per block
-constexpr int INFER_BLOCKS = 1536;  // 6 wavefronts per SIMD; they loop over 32-sample tiles
+constexpr int INFER_BLOCKS = 1024;  // 4 per CU: all resident at 4 wavefronts per SIMD (<= 128 VGPRs); they loop over 32-sample tiles
 
 // GUESS: start cells from positions (cell < 0 or no cell array; others get
 // -1, and tiles where nobody needs one skip the network).  Otherwise the
@@ -191,31 +208,41 @@ __global__ __launch_bounds__(64 * INFER_WAVES, 4) void k_mlp_infer(int64_t n, co
                                                                 const float *__restrict__ xin, MlpWeights Wt,
                                                                 MlpNorm nm, int IM, int JM, int KM,
                                                                 int32_t *__restrict__ guess, float *__restrict__ yout) {
-  __shared__ float4 w1b[H];          // layer 1: (w0, w1, w2, bias) of each hidden unit
-  __shared__ bf16x4 w2s[2][8][64];   // layer 2 A-operand fragments, by (column tile, k block, lane)
-  __shared__ float b2s[H];           // layer 2 biases
-  __shared__ bf16x4 w3s[8][64];      // layer 3 A-operand fragments (rows 3..31 zero), by (k block, lane)
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r32 = l & 31, h = l >> 5;
-  if (threadIdx.x < H) {
-    const int q = threadIdx.x;
-    w1b[q] = make_float4(bf2f(Wt.wb[OFF_W1 + 3 * q]), bf2f(Wt.wb[OFF_W1 + 3 * q + 1]),
-                         bf2f(Wt.wb[OFF_W1 + 3 * q + 2]), Wt.p[OFF_B1 + q]);
-    b2s[q] = Wt.p[OFF_B2 + q];
-  }
-  for (int e = threadIdx.x; e < 2 * 8 * 64; e += blockDim.x) {
-    const int t = e / 512, kk = (e / 64) % 8, ln = e % 64;
-    bf16x4 f;
+  // layer 1 A operands: A(m, k) with m = r32 and k = h (first product) or 2 + h (second; k = 3 is the bias)
+  float w1a[2][2];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) f[v] = (short)Wt.wb[OFF_W2 + (32 * t + (ln & 31)) * H + 8 * kk + 4 * (ln >> 5) + v];
-    w2s[t][kk][ln] = f;
+  for (int t = 0; t < 2; ++t) {
+    const int q = 32 * t + r32;
+    w1a[t][0] = bf2f(Wt.wb[OFF_W1 + 3 * q + h]);
+    w1a[t][1] = h == 0 ? bf2f(Wt.wb[OFF_W1 + 3 * q + 2]) : Wt.p[OFF_B1 + q];
   }
-  for (int e = threadIdx.x; e < 8 * 64; e += blockDim.x) {
-    const int kk = e / 64, ln = e % 64, c = ln & 31;
-    bf16x4 f;
+  // layer 2 A operands (registers) and layer 3 A operands (LDS, by K block and lane):
+  // A(m, k) = W[32t + m][infer_unit(kb, k)], m = r32, k = 8h + j; layer 3 rows 3..31 are zero
+  __shared__ bf16x8 w3s[4][64];
+  __shared__ float4 b2s[H / 4];  // layer 2 biases
+  bf16x8 w2a[2][4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) f[v] = c < 3 ? (short)Wt.wb[OFF_W3 + c * H + 8 * kk + 4 * (ln >> 5) + v] : (short)0;
-    w3s[kk][ln] = f;
+  for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int u = infer_unit(kb, 8 * h + j);
+      w2a[0][kb][j] = __builtin_bit_cast(__bf16, Wt.wb[OFF_W2 + r32 * H + u]);
+      w2a[1][kb][j] = __builtin_bit_cast(__bf16, Wt.wb[OFF_W2 + (32 + r32) * H + u]);
+    }
+  if (wv == 0) {
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+      bf16x8 f;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        f[j] = __builtin_bit_cast(__bf16, r32 < 3 ? Wt.wb[OFF_W3 + r32 * H + infer_unit(kb, 8 * h + j)] : (uint16_t)0);
+      w3s[kb][l] = f;
+    }
+    if (l < H / 4)
+      b2s[l] = make_float4(Wt.p[OFF_B2 + 4 * l], Wt.p[OFF_B2 + 4 * l + 1], Wt.p[OFF_B2 + 4 * l + 2],
+                           Wt.p[OFF_B2 + 4 * l + 3]);
   }
   __syncthreads();
   const float b30 = Wt.p[OFF_B3], b31 = Wt.p[OFF_B3 + 1], b32 = Wt.p[OFF_B3 + 2];
@@ -245,35 +272,42 @@ __global__ __launch_bounds__(64 * INFER_WAVES, 4) void k_mlp_infer(int64_t n, co
         x2 = xin[3 * p + 2];
       }
     }
-    // layer 1 (VALU), straight into the B operand of layer 2
-    bf16x4 a1f[8];
+    // layer 1: B(k, n) = x_k of sample n, B(3, n) = 1
+    const float xb0 = h == 0 ? x0 : x1, xb1 = h == 0 ? x2 : 1.0f;
+    bf16x8 a1[4];
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk)
+    for (int t = 0; t < 2; ++t) {
+      f32x16 z = {};
+      z = __builtin_amdgcn_mfma_f32_32x32x2f32(w1a[t][0], xb0, z, 0, 0, 0);
+      z = __builtin_amdgcn_mfma_f32_32x32x2f32(w1a[t][1], xb1, z, 0, 0, 0);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const float4 w = w1b[8 * kk + 4 * h + v];
-        const float z = __builtin_fmaf(w.x, x0, __builtin_fmaf(w.y, x1, __builtin_fmaf(w.z, x2, w.w)));
-        a1f[kk][v] = bf16_rne(act(z));
-      }
-    // layer 2 (MFMA): column tile t holds hidden units 32t..32t+31
-    bf16x4 a2f[8];
+      for (int r = 0; r < 16; ++r) a1[2 * t + r / 8][r % 8] = (__bf16)act(z[r]);
+    }
+    // layer 2
+    bf16x8 a2[4];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       f32x16 acc;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) acc[r] = b2s[32 * t + 8 * (r / 4) + 4 * h + (r % 4)];
+      for (int g = 0; g < 4; ++g) {  // registers 4g..4g+3: units 32t + 8g + 4h + 0..3
+        const float4 b = b2s[8 * t + 2 * g + h];
+        acc[4 * g] = b.x;
+        acc[4 * g + 1] = b.y;
+        acc[4 * g + 2] = b.z;
+        acc[4 * g + 3] = b.w;
+      }
 #pragma unroll
-      for (int kk = 0; kk < 8; ++kk) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w2s[t][kk][l], a1f[kk], acc, 0, 0, 0);
+      for (int kb = 0; kb < 4; ++kb) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2a[t][kb], a1[kb], acc, 0, 0, 0);
 #pragma unroll
-      for (int r = 0; r < 16; ++r) a2f[4 * t + r / 4][r % 4] = bf16_rne(act(acc[r]));
+      for (int r = 0; r < 16; ++r) a2[2 * t + r / 8][r % 8] = (__bf16)act(acc[r]);
     }
-    // layer 3 (MFMA): rows 0..2 of the result (registers 0..2 of lanes 0..31)
+    // layer 3: rows 0..2 of the result (registers 0..2 of lanes 0..31) are the outputs
     f32x16 acc3 = {};
-    acc3[0] = b30;  // rows 0..2 of the result: the outputs, accumulated onto their biases
+    acc3[0] = b30;
     acc3[1] = b31;
     acc3[2] = b32;
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) acc3 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w3s[kk][l], a2f[kk], acc3, 0, 0, 0);
+    for (int kb = 0; kb < 4; ++kb) acc3 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w3s[kb][l], a2[kb], acc3, 0, 0, 0);
     if (h == 0 && in) {
       const float y0 = acc3[0], y1 = acc3[1], y2 = acc3[2];
       if (GUESS) {

@@ -53,9 +53,9 @@ def test_mlp_forward_matches_oracle(ctx, oracle):
 
 
 def test_mlp_forward_many_tiles_per_wavefront(ctx, oracle):
-    """Above 1536 blocks x 4 wavefronts x 32 = 196,608 samples each wavefront
+    """Above 1024 blocks x 4 wavefronts x 32 = 131,072 samples each wavefront
     of the inference kernel loops over several 32-sample tiles; this size
-    gives three or four tiles per wavefront and a ragged last tile."""
+    gives four or five tiles per wavefront and a ragged last tile."""
     _say("mlp_forward many tiles")
     p = M.initial_params(8)
     ctx.mlp_set_params(p)
