@@ -1,8 +1,11 @@
 // mlp.hip -- the learned initial guess for the walking search (gfx950).
 //
-// A 3 -> 64 -> 64 -> 3 MLP with the hard-swish activation
-// z clamp(z/6 + 1/2, 0, 1) (Howard et al., "Searching for MobileNetV3", 2019,
-// section 5.2; the same function in training, inference and the oracle) maps normalised physical
+// A 3 -> 64 -> 64 -> 3 MLP with GELU in its hard (piecewise-linear) form,
+// z Phi(z) with the normal CDF Phi replaced by its tangent at 0 clipped to
+// [0, 1]: z clamp(z / sqrt(2 pi) + 1/2, 0, 1) (the construction of
+// hard-sigmoid and hard-swish, Howard et al., "Searching for MobileNetV3",
+// 2019, section 5.2, applied to GELU; the same function in training,
+// inference and the oracle) maps normalised physical
 // coordinates to the fractional logical position ((i+a1)/IM, (j+a2)/JM,
 // (k+a3)/KM).  Weights are BF16, products accumulate in FP32; layer 1 runs on
 // the VALU (K = 3), layers 2 and 3 and the large backward products on MFMA
@@ -45,11 +48,11 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-// Hard-swish z t(z), gate t = clamp(fma(z, c, 1/2), 0, 1) with c = fl32(1/6):
-// one fused multiply-add with the VOP3 clamp and one multiply, no
-// transcendental.  Derivative t + c z where the gate is strictly inside
-// (0, 1), t (0 or 1) where it is clamped.
-constexpr float kActC = 1.0f / 6.0f;
+// Hard GELU z t(z), gate t = clamp(fma(z, c, 1/2), 0, 1) with
+// c = fl32(1/sqrt(2 pi)) = Phi'(0): one fused multiply-add with the VOP3
+// clamp and one multiply, no transcendental.  Derivative t + c z where the
+// gate is strictly inside (0, 1), t (0 or 1) where it is clamped.
+constexpr float kActC = 0.3989422804014327f;
 __device__ __forceinline__ float act_gate(float z) {
   return __builtin_amdgcn_fmed3f(__builtin_fmaf(z, kActC, 0.5f), 0.0f, 1.0f);
 }

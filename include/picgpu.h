@@ -346,7 +346,7 @@ int picgpu_write_vtk(picgpu_ctx *ctx, const char *prefix, int64_t step);
 int picgpu_seed_particles(picgpu_ctx *ctx, int64_t n, int mode, int inlet_face, int64_t seed, int64_t pid_base);
 int picgpu_reinit_on_inlet(picgpu_ctx *ctx, int inlet_face, int64_t seed);
 
-/* ---- MLP initial guess (3 -> 64 -> 64 -> 3, hard-swish, BF16 weights, FP32 accumulation) ---- */
+/* ---- MLP initial guess (3 -> 64 -> 64 -> 3, hard GELU, BF16 weights, FP32 accumulation) ---- */
 #define PICGPU_MLP_HIDDEN 64
 /* number of FP32 parameters: W1[64][3] b1[64] W2[64][64] b2[64] W3[3][64] b3[3] */
 int64_t picgpu_mlp_param_count(void);

@@ -614,10 +614,12 @@ static double bf16_round(double v) {
   memcpy(&f, &u, 4);
   return (double)f;
 }
-/* Hard-swish z t(z) with the gate t = clamp(c z + 1/2, 0, 1), c = the binary32
-   value of 1/6 (Howard et al. 2019, "Searching for MobileNetV3", sec. 5.2).
+/* GELU z Phi(z) in its hard form: Phi replaced by its tangent at 0 clipped
+   to [0, 1], the gate t = clamp(c z + 1/2, 0, 1) with c = the binary32 value
+   of 1/sqrt(2 pi) = Phi'(0) (the hard-sigmoid construction of Howard et al.
+   2019, "Searching for MobileNetV3", sec. 5.2).
    Derivative: t + c z where 0 < c z + 1/2 < 1, else t (0 or 1). */
-static const double act_c = (double)(1.0f / 6.0f);
+static const double act_c = (double)0.3989422804014327f;
 static double act_gate(double z) {
   const double u = z * act_c + 0.5;
   return u < 0.0 ? 0.0 : (u > 1.0 ? 1.0 : u);

@@ -182,7 +182,7 @@ double ref_drand48(uint64_t *state);        /* POSIX drand48 */
 void ref_seed_particles(int64_t IM, int64_t JM, int64_t KM, const double *coor, int64_t n, int mode, int face,
                         int64_t seed, double *pos);
 
-/* ---- SPEC: MLP initial guess, 3 -> H -> H -> 3, hard-swish activation ---- */
+/* ---- SPEC: MLP initial guess, 3 -> H -> H -> 3, GELU in its hard form z clamp(z/sqrt(2 pi) + 1/2, 0, 1) ---- */
 /* params layout: W1[H][3], b1[H], W2[H][H], b2[H], W3[3][H], b3[3] (row = output) */
 int64_t ref_mlp_param_count(int H);
 /* forward; when round_bf16 != 0, weights and every matmul input are rounded to

@@ -1,7 +1,7 @@
 """GPU MLP (learned initial guess) against the CPU FP64 oracle.
 
 The oracle applies the device's rounding rule (BF16 weights and BF16 matmul
-inputs a1, a2, g2; FP64 elsewhere) and the same hard-swish activation, so
+inputs a1, a2, g2; FP64 elsewhere) and the same hard-GELU activation, so
 the remaining differences come from FP32 accumulation: a rare one-ulp flip
 of a BF16-rounded activation moves one output by ~1e-3.  Tolerances are
 written per test.

@@ -377,7 +377,7 @@ def test_mlp_gradient_matches_finite_differences(oracle, w1):
 
 
 def test_mlp_forward_by_hand(oracle):
-    """One hidden unit through each layer, written out, with the hard-swish
+    """One hidden unit through each layer, written out, with the hard GELU's
     gate inside (0, 1) on the way in and saturated at 1 and at 0 on two
     more units."""
     H = 64
@@ -389,7 +389,7 @@ def test_mlp_forward_by_hand(oracle):
     W3, b3 = p[off:off + 3 * H].reshape(3, H), p[off + 3 * H:]
     W1[0] = [1.0, -2.0, 0.5]
     b1[0] = 0.25
-    W2[3, 0] = 2.0
+    W2[3, 0] = 1.0
     b2[3] = -0.5
     W3[1, 3] = 3.0
     b3[:] = [0.1, 0.2, 0.3]
@@ -397,11 +397,11 @@ def test_mlp_forward_by_hand(oracle):
     W3[0, 5] = 0.5
     b2[6] = -7.0   # z = -7: gate clamped at 0, act = 0
     W3[2, 6] = 4.0
-    c = float(np.float32(1.0) / np.float32(6.0))  # the binary32 value of 1/6
-    act = lambda z: z * min(max(c * z + 0.5, 0.0), 1.0)  # hard-swish
+    c = float(np.float32(1.0 / math.sqrt(2.0 * math.pi)))  # binary32 value of Phi'(0)
+    act = lambda z: z * min(max(c * z + 0.5, 0.0), 1.0)  # GELU, hard form
     x = np.array([0.3, -0.1, 0.7])
     z1 = 0.3 + 0.2 + 0.35 + 0.25
-    z2 = 2.0 * act(z1) - 0.5
+    z2 = act(z1) - 0.5
     assert 0 < c * z1 + 0.5 < 1 and 0 < c * z2 + 0.5 < 1
     y = oracle.mlp_forward(p, x[None], round_bf16=False)[0]
     np.testing.assert_allclose(y, [0.1 + 0.5 * 7.0, 0.2 + 3.0 * act(z2), 0.3], rtol=1e-14)
