@@ -1453,22 +1453,20 @@ __global__ __launch_bounds__(kBlock) void k_iota(int64_t n, const int32_t *__res
 // Puts every bucket idx[kstart[key] .. kstart[key+1]) in ascending PID order,
 // the summation order of deposition (and of the oracle).  Buckets of up to 8
 // particles are ranked in registers; longer ones, rare at a few particles per
-// cell, are insertion-sorted in place.
-__global__ __launch_bounds__(kBlock) void k_bucket_pid_order(int64_t nkeys, const int32_t *__restrict__ kstart,
-                                                             int32_t *__restrict__ idx,
-                                                             const int64_t *__restrict__ pid) {
-  const int64_t key = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (key >= nkeys) return;
-  const int32_t s = kstart[key], e = kstart[key + 1];
+// cell, are insertion-sorted in place.  The PIDs are read as T: the 32-bit
+// copy pid32 when every PID lies in [0, 2^31) (*wide == 0; the same order,
+// half the bytes), the 64-bit pid otherwise.
+template <typename T>
+__device__ __forceinline__ void bucket_pid_order(int32_t s, int32_t e, int32_t *__restrict__ idx,
+                                                 const T *__restrict__ pid) {
   const int32_t L = e - s;
-  if (L < 2) return;
   if (L <= 8) {
     int32_t q[8];
-    int64_t v[8];
+    T v[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) q[t] = t < L ? idx[s + t] : 0;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) v[t] = t < L ? pid[q[t]] : INT64_MAX;
+    for (int t = 0; t < 8; ++t) v[t] = t < L ? pid[q[t]] : (sizeof(T) == 4 ? (T)INT32_MAX : (T)INT64_MAX);
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
       int r = 0;
@@ -1480,7 +1478,7 @@ __global__ __launch_bounds__(kBlock) void k_bucket_pid_order(int64_t nkeys, cons
   }
   for (int32_t t = s + 1; t < e; ++t) {
     const int32_t x = idx[t];
-    const int64_t px = pid[x];
+    const T px = pid[x];
     int32_t u = t - 1;
     while (u >= s && pid[idx[u]] > px) {
       idx[u + 1] = idx[u];
@@ -1488,6 +1486,20 @@ __global__ __launch_bounds__(kBlock) void k_bucket_pid_order(int64_t nkeys, cons
     }
     idx[u + 1] = x;
   }
+}
+__global__ __launch_bounds__(kBlock) void k_bucket_pid_order(int64_t nkeys, const int32_t *__restrict__ kstart,
+                                                             int32_t *__restrict__ idx,
+                                                             const int64_t *__restrict__ pid,
+                                                             const int32_t *__restrict__ pid32,
+                                                             const int32_t *__restrict__ wide) {
+  const int64_t key = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (key >= nkeys) return;
+  const int32_t s = kstart[key], e = kstart[key + 1];
+  if (e - s < 2) return;
+  if (*wide == 0)
+    bucket_pid_order<int32_t>(s, e, idx, pid32);
+  else
+    bucket_pid_order<int64_t>(s, e, idx, pid);
 }
 
 // Face deposition (no reference counterpart).  A particle in cell (i,j,k)
@@ -1733,7 +1745,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_swarm(int64_t m_max, const in
                                                          double *__restrict__ pos2, R *__restrict__ vel2,
                                                          R *__restrict__ w2, int32_t *__restrict__ cell2,
                                                          int32_t *__restrict__ status2, int64_t *__restrict__ pid2,
-                                                         R *__restrict__ P2) {
+                                                         R *__restrict__ P2, const int32_t *__restrict__ pid32,
+                                                         int32_t *__restrict__ pid32_2) {
   constexpr int NP = kGatherPer;
   const int64_t m = dev_count(m_max, m_dev);
   const int64_t q0 = (int64_t)blockIdx.x * (kBlock * NP) + threadIdx.x;
@@ -1772,7 +1785,22 @@ __global__ __launch_bounds__(kBlock) void k_gather_swarm(int64_t m_max, const in
     if (mode != GATHER_STEP) status2[q] = st[u];
     pid2[q] = id[u];
     P2[q] = pp[u];
+    if (pid32) pid32_2[q] = (int32_t)id[u];  // the low 32 bits, as k_pid32 writes them
   }
+}
+
+// pid32[q] = the low 32 bits of pid[q]; *wide |= 1 when some PID lies
+// outside [0, 2^31), where the 32-bit order is not the PID order.
+__global__ __launch_bounds__(kBlock) void k_pid32(int64_t n, const int64_t *__restrict__ pid,
+                                                  int32_t *__restrict__ pid32, int32_t *__restrict__ wide) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool out = false;
+  if (q < n) {
+    const int64_t v = pid[q];
+    pid32[q] = (int32_t)v;
+    out = v < 0 || v > (int64_t)INT32_MAX;
+  }
+  if (__ballot(out) != 0 && (threadIdx.x & 63) == 0) atomicOr(wide, 1);
 }
 
 }  // namespace picgpu

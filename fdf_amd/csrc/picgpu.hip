@@ -131,10 +131,16 @@ static int ensure_swarm(picgpu_ctx *c, SwarmBufs &s, int64_t n) {
   PG_CHECK(s.status.ensure(m * sizeof(int32_t)));
   PG_CHECK(s.pid.ensure(m * sizeof(int64_t)));
   PG_CHECK(s.P.ensure(m * c->rsize));
+  PG_CHECK(s.pid32.ensure(m * sizeof(int32_t)));
   return PICGPU_OK;
 }
 
-static void swap_swarm(picgpu_ctx *c) { std::swap(c->sw, c->alt); }
+// Every move of the swarm arrays goes through here; only the key-order
+// compaction of the fused steps moves pid32 along (and says so after the swap).
+static void swap_swarm(picgpu_ctx *c) {
+  std::swap(c->sw, c->alt);
+  c->pid32_valid = false;
+}
 
 static int need_grid(picgpu_ctx *c) {
   if (!c->have_grid) {
@@ -284,7 +290,9 @@ int picgpu_destroy(picgpu_ctx *c) {
     s->status.release();
     s->pid.release();
     s->P.release();
+    s->pid32.release();
   }
+  c->pid_wide.release();
   (void)hipStreamDestroy(c->stream);
   delete c;
   return PICGPU_OK;
@@ -374,6 +382,7 @@ int picgpu_init_particles_device(picgpu_ctx *c, int64_t n, const double *d_pos, 
   c->n = n;
   c->sorted = c->keyed = c->key_ordered = c->compact_pending = false;
   c->swarm_invalid = false;
+  c->pid32_valid = false;
   if (n == 0) return PICGPU_OK;
   int r = dispatch_real(c, [&](auto *tag) -> int {
     using R = std::remove_pointer_t<decltype(tag)>;
@@ -431,6 +440,7 @@ int picgpu_upload_particles(picgpu_ctx *c, int64_t n, const double *pos, const d
   c->n = n;
   c->sorted = c->keyed = c->key_ordered = c->compact_pending = false;
   c->swarm_invalid = false;
+  c->pid32_valid = false;
   if (n == 0) return PICGPU_OK;
   PG_HIP(hipMemcpy(c->sw.pos.ptr, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice));
   PG_HIP(hipMemcpy(c->sw.pid.ptr, pid, sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice));
@@ -885,6 +895,7 @@ static int build_key_index(picgpu_ctx *c, const int32_t *guess) {
 static int compact_by_key(picgpu_ctx *c, int64_t live, int mode, const int32_t *live_dev = nullptr) {
   PG_CHECK(ensure_swarm(c, c->alt, c->n));
   int32_t *idx = c->kidx[c->kcur].as<int32_t>();
+  const bool move32 = c->pid32_valid;  // the 32-bit PID copy moves with the swarm
   if (live > 0) {
     int r = dispatch_real(c, [&](auto *tag) -> int {
       using R = std::remove_pointer_t<decltype(tag)>;
@@ -893,13 +904,15 @@ static int compact_by_key(picgpu_ctx *c, int64_t live, int mode, const int32_t *
                 c->sw.vel.as<R>(), c->sw.w.as<R>(), c->sw.cell.as<int32_t>(), c->sw.status.as<int32_t>(),
                 c->sw.pid.as<int64_t>(), c->sw.P.as<R>(), c->alt.pos.as<double>(), c->alt.vel.as<R>(),
                 c->alt.w.as<R>(), c->alt.cell.as<int32_t>(), c->alt.status.as<int32_t>(), c->alt.pid.as<int64_t>(),
-                c->alt.P.as<R>());
+                c->alt.P.as<R>(), move32 ? c->sw.pid32.as<int32_t>() : (const int32_t *)nullptr,
+                c->alt.pid32.as<int32_t>());
       return PICGPU_OK;
     });
     PG_CHECK(r);
     PG_LAUNCH(c, "iota", nblocks(live), kBlock, k_iota, live, live_dev, idx);
   }
   swap_swarm(c);
+  c->pid32_valid = move32;
   if (!live_dev) c->n = live;
   c->key_ordered = true;
   return PICGPU_OK;
@@ -1091,9 +1104,19 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, int step, boo
               c->sw.cell.as<int32_t>(),
               c->kslot.as<int32_t>(), kbase, CellIdx{c->IM, c->JM, grid_view(c).rI, grid_view(c).rIJ}, tm,
               c->kstart[nxt].as<int32_t>(), c->kidx[nxt].as<int32_t>());
-  if (n > 0 && (flags & (PICGPU_STEP_DEPOSIT_CELLS | PICGPU_STEP_DEPOSIT_FACES)))
+  if (n > 0 && (flags & (PICGPU_STEP_DEPOSIT_CELLS | PICGPU_STEP_DEPOSIT_FACES))) {
+    if (!c->pid32_valid) {
+      // the 32-bit PID copy, made once after the swarm changed; compaction moves it
+      PG_CHECK(c->pid_wide.ensure(sizeof(int32_t)));
+      PG_HIP(hipMemsetAsync(c->pid_wide.ptr, 0, sizeof(int32_t), c->stream));
+      PG_LAUNCH(c, "pid32", nblocks(n), kBlock, k_pid32, n, c->sw.pid.as<int64_t>(), c->sw.pid32.as<int32_t>(),
+                c->pid_wide.as<int32_t>());
+      c->pid32_valid = true;
+    }
     PG_LAUNCH(c, "bucket_pid_order", nblocks(nkeys), kBlock, k_bucket_pid_order, nkeys, c->kstart[nxt].as<int32_t>(),
-              c->kidx[nxt].as<int32_t>(), c->sw.pid.as<int64_t>());
+              c->kidx[nxt].as<int32_t>(), c->sw.pid.as<int64_t>(), c->sw.pid32.as<int32_t>(),
+              c->pid_wide.as<int32_t>());
+  }
   if (hist) PG_CHECK(hist_read(c));
   // every survivor of a fused step is located: the next index has no unlocated bucket
   c->kcur = nxt;

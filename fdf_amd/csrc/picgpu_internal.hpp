@@ -68,6 +68,7 @@ struct SwarmBufs {
   DevBuf status;  // int32 [n]
   DevBuf pid;     // int64 [n]
   DevBuf P;       // real [n]
+  DevBuf pid32;   // int32 [n]: the PID's low 32 bits, while picgpu_ctx::pid32_valid
 };
 
 enum Counter : int {
@@ -158,6 +159,11 @@ struct picgpu_ctx {
   int kcur = 0;              // index in use
   bool keyed = false;        // kidx[kcur] / kstart[kcur] describe the current swarm
   bool key_ordered = false;  // the swarm arrays are (nearly) in key order: compacted since the last change
+  // sw.pid32 holds the low 32 bits of sw.pid, and pid_wide (int32 on the
+  // device) is nonzero if some PID lies outside [0, 2^31): the PID order of
+  // the buckets then compares the 64-bit PIDs
+  bool pid32_valid = false;
+  picgpu::DevBuf pid_wide;
   // a step call ended between compactions: kidx[kcur] lists the n_live live
   // particles and the arrays (length n) still hold removed ones; settle_swarm
   // compacts them before anything else reads or changes the arrays

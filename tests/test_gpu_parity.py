@@ -462,7 +462,7 @@ def test_long_fused_run_matches_staged(swarm_cls, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-@pytest.mark.parametrize("case", ["warped", "crowded"])
+@pytest.mark.parametrize("case", ["warped", "crowded", "crowded_wide_pid"])
 def test_steps_deposit_the_state_they_start_from(swarm_cls, oracle, case, precision):
     """Inside a call every step deposits the state it starts from (the
     previous step's deposition, made while the step walks its particles);
@@ -470,7 +470,10 @@ def test_steps_deposit_the_state_they_start_from(swarm_cls, oracle, case, precis
     It equals the oracle's deposition of the state one step before the end:
     bit for bit in FP64, within 1e-5 in FP32 (of the FP32 run's own state).
     The crowded case has thousands of particles per tile, more than the
-    kernel stages in LDS."""
+    kernel stages in LDS.  Its wide-PID form gives PIDs above 2^32 whose low
+    32 bits are in another order than the PIDs, so the buckets must be put
+    in the order of the 64-bit PIDs (the 32-bit copy serves only PIDs in
+    [0, 2^31))."""
     _say("deposit_start_state", case, precision)
     if case == "warped":
         d = W.config(5, scale=0.02)
@@ -485,12 +488,22 @@ def test_steps_deposit_the_state_they_start_from(swarm_cls, oracle, case, precis
              "dt": 1.5 / (np.abs(ucat).max() * max(IM, JM, KM))}
     IM, JM, KM = d["IM"], d["JM"], d["KM"]
     nsteps = 4
+    npart = d["pos"].shape[0]
+    pid = None
+    if case == "crowded_wide_pid":
+        rng = np.random.default_rng(34)
+        pid = (rng.permutation(npart).astype(np.int64) << 32) | rng.integers(0, 1 << 31, npart)
+        lo = pid & 0xFFFFFFFF
+        assert (np.argsort(lo) != np.argsort(pid)).any()
 
     def run(n, faces_of_start):
         with swarm_cls(precision=precision) as g:
             g.upload_grid(IM, JM, KM, d["coor"])
             g.upload_ucat(d["ucat"])
-            g.upload_particles(d["pos"])
+            if pid is None:
+                g.upload_particles(d["pos"])
+            else:
+                g.upload_particles(d["pos"], pid=pid)
             g.LocateAllParticlesInGrid()
             g.InterpolateAllFieldsToSwarm()
             g.AdvanceSimulation(d["dt"], n, rk2=True, deposit_cells=False, deposit_faces=True,
@@ -503,9 +516,10 @@ def test_steps_deposit_the_state_they_start_from(swarm_cls, oracle, case, precis
     assert len(after["pid"]) < len(before["pid"]) or case == "warped"
     ref = oracle.deposit_faces(IM, JM, KM, before["cell"], before["weight"], before["vel"], before["pid"])
     if precision == "fp64":
-        s, _ = _run_oracle_steps(oracle, d, nsteps - 1, True)
-        for k in ("pid", "cell", "pos", "vel", "weight"):
-            assert np.array_equal(before[k], s[k]), k
+        if pid is None:
+            s, _ = _run_oracle_steps(oracle, d, nsteps - 1, True)
+            for k in ("pid", "cell", "pos", "vel", "weight"):
+                assert np.array_equal(before[k], s[k]), k
         for a, b in zip(faces, ref):
             assert np.array_equal(a, b)
     else:
