@@ -117,7 +117,7 @@ def measured_traffic(workload: str, variant: str, kernel: str, later_share: floa
 
 def calibration_copy(nbytes: int = 1 << 30) -> float:
     """A plain device copy of known size, so that counter runs of this command
-    can scale FETCH_SIZE / WRITE_SIZE to bytes; then five more, timed with
+    can check their byte counts against it; then five more, timed with
     events, for the measured copy bandwidth (bytes read + written per second,
     GB/s), the roofline's second denominator."""
     import torch
@@ -757,9 +757,9 @@ def run_relocate(d: dict, precision: str, use_mlp: bool, steps: int, warmup: int
         g.upload_grid_device(IM, JM, KM, d["coor"].data_ptr())
         g.upload_ucat_device(d["ucat"].data_ptr())
         if use_mlp:
-            # longer training than the library default for the bent tube: the
-            # mean walk from the guess goes from 7.17 to 5.40 iterations for
-            # about 3.8 s of training (outside the timed locates)
+            # longer training than the library default for the bent tube
+            # (12,000 iterations at batch 16,384, about 1 s, outside the timed
+            # locates): the mean walk from the guess is about 3.4 iterations
             res["mlp_train"] = M.train_from_walk(g, d, iters=12000, batch=16384, lr=2e-3)
         g.walk_stats(True)
         secs, iters, hist = 0.0, 0, None
