@@ -443,6 +443,9 @@ def _sub_block(d: dict, dims) -> dict:
     IM, JM, KM = d["IM"], d["JM"], d["KM"]
     ni, nj, nk = (min(a, b) for a, b in zip(dims, (IM, JM, KM)))
     i0, j0, k0 = (IM - ni) // 2, (JM - nj) // 2, (KM - nk) // 2
+    if (ni, nj, nk) == (IM, JM, KM):  # the whole workload (any grid shape, every particle)
+        return {"IM": IM, "JM": JM, "KM": KM, "origin": (0, 0, 0), "coor": d["coor"].cpu().numpy().reshape(-1, 3),
+                "ucat": d["ucat"].cpu().numpy().reshape(-1, 3), "pos": d["pos_dev"].cpu().numpy()}
     c = d["coor"].reshape(KM + 1, JM + 1, IM + 1, 3)[k0:k0 + nk + 1, j0:j0 + nj + 1, i0:i0 + ni + 1]
     u = torch.zeros((nk + 1, nj + 1, ni + 1, 3), dtype=torch.float64, device=d["ucat"].device)
     u[:nk, :nj, :ni] = d["ucat"].reshape(KM + 1, JM + 1, IM + 1, 3)[k0:k0 + nk, j0:j0 + nj, i0:i0 + ni]
@@ -472,15 +475,16 @@ def _cpu_rate(o, blk: dict, d: dict, threads: int, runs: int, steps: int) -> dic
     s0 = {"pos": blk["pos"].copy(), "vel": o.interp(IM, JM, KM, corner, cell, w), "cell": cell, "weight": w,
           "status": st, "pid": np.arange(n, dtype=np.int64), "P": np.full(n, 101325.0)}
     faces = "faces" in d["deposit"]
+    cells = "cells" in d["deposit"] or not faces
 
     def one_step(s):
         c2 = o.center_to_corner(IM, JM, KM, blk["ucat"])
         o.step(IM, JM, KM, coor, c2, s, d["dt"], rk2=d["rk2"], bbox=bbox)
-        if faces:
-            o.deposit_faces(IM, JM, KM, s["cell"], s["weight"], s["vel"], s["pid"])
-        else:
+        if cells:
             cnt = o.count_cells(IM, JM, KM, s["cell"])
             o.scatter_P(IM, JM, KM, s["cell"], s["P"], s["pid"], cnt)
+        if faces:
+            o.deposit_faces(IM, JM, KM, s["cell"], s["weight"], s["vel"], s["pid"])
 
     rates, secs = [], []
     for _ in range(runs):
@@ -518,12 +522,14 @@ def cpu_baseline(d: dict, all_block=(128, 128, 64), one_block=(128, 128, 64), ru
     r_one = _cpu_rate(o, blk_one, d, 1, runs, steps)
 
     def desc(r):
+        if list(r["cells"]) == [d["IM"], d["JM"], d["KM"]]:
+            return f"the whole {d['IM']}x{d['JM']}x{d['KM']} grid with its {r['particles']} particles"
         return (f"{r['cells'][0]}x{r['cells'][1]}x{r['cells'][2]} cells from the middle of the "
                 f"{d['IM']}x{d['JM']}x{d['KM']} grid with its {r['particles']} particles")
 
     return {"value": r_all["value"], "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": f"config {d['config']} ({'RK2' if d['rk2'] else 'Euler'}, "
-                      f"{'face' if 'faces' in d['deposit'] else 'cell'} deposition, centre->corner each step): "
+                      f"deposition {d['deposit']}, centre->corner each step): "
                       f"{desc(r_all)}; median of {runs} runs of {steps} steps after a warm-up step; "
                       f"FP64 oracle -O3 x86-64-v3 OpenMP; omits the reference's PETSc view overheads",
             "one_core": {"value": r_one["value"], "cores": 1, "sample": desc(r_one), "runs": r_one["runs"]},
@@ -573,7 +579,10 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--variants", default="fp32_mlp,fp64")
     ap.add_argument("--cpu-block", default="128,128,64",
-                    help="cells of the CPU baseline's sub-block (timed with every core and with one)")
+                    help="cells of the CPU baseline's sub-block, timed with every core (a block at least "
+                         "as large as the grid: the whole workload)")
+    ap.add_argument("--cpu-one-block", default=None,
+                    help="cells of the sub-block timed with one core (default: --cpu-block)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--call-steps", type=int, default=0,
                     help="time the steps as picgpu_step calls of this many steps each (0: one call)")
@@ -655,7 +664,8 @@ def main():
     if not args.no_cpu_baseline:
         try:
             blk = tuple(int(v) for v in args.cpu_block.split(","))
-            cpu = cpu_baseline(d, blk, blk)
+            blk1 = tuple(int(v) for v in (args.cpu_one_block or args.cpu_block).split(","))
+            cpu = cpu_baseline(d, blk, blk1)
             log(f"cpu baseline: {cpu['value']:.4g} particle-steps/s on {cpu['cores']} cores")
         except Exception as e:
             log("cpu baseline failed:", e)
