@@ -1060,7 +1060,8 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, int step, boo
   const bool dep = !wu && faces && step > 0;
   alignas(16) unsigned char fo_buf[sizeof(FaceOut<double>)];
   if (dep) PG_CHECK(face_out(c, fo_buf));
-  if (n > 0) {
+  // an empty swarm still deposits (zero faces) when the step deposits
+  if (n > 0 || dep) {
     int r = dispatch_real(c, [&](auto *tag) -> int {
       using R = std::remove_pointer_t<decltype(tag)>;
       StepArgs<R> a;
@@ -1084,7 +1085,7 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, int step, boo
         PG_LAUNCH(c, name, nt, kBlock, (k_step_tiled<R, EX, RK, WU>), a, idx, kstart, kbase);
         if (fresh && u1 > u0)
           PG_LAUNCH(c, name_list, nblocks(u1 - u0), kBlock, (k_step_list<R, EX, RK>), a, idx, u0, u1);
-        PG_LAUNCH(c, "step_deferred", dblocks, kBlock, (k_step_deferred<R, EX, RK>), a);
+        if (n > 0) PG_LAUNCH(c, "step_deferred", dblocks, kBlock, (k_step_deferred<R, EX, RK>), a);
         return PICGPU_OK;
       };
       using T = std::true_type;
