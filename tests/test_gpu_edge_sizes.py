@@ -140,3 +140,26 @@ def test_small_sizes_against_oracle(swarm_cls, oracle, precision, path, case):
             a4, b4 = a.reshape(-1, 4), b.reshape(-1, 4)
             assert np.abs(a4[:, 0] - b4[:, 0]).max() <= FP32_TOL * max(1.0, np.abs(b4[:, 0]).max())
             assert np.abs(a4[:, 1:] - b4[:, 1:]).max() <= FP32_TOL * max(1.0, np.abs(b4[:, 1:]).max())
+
+
+def test_sizes_beyond_32_bit_indices_are_rejected(swarm_cls):
+    """Cell numbers and particle numbers are 32-bit on the device: a grid of
+    2^31 - 2 cells or more, and 2^31 - 1 particles or more, are refused with
+    PICGPU_E_UNSUPPORTED before anything is read from the caller's arrays
+    (here one-element stand-ins)."""
+    _say("size limits")
+    one = np.zeros(3)
+    with swarm_cls(precision="fp64") as g:
+        ptr = one.ctypes.data_as(L.C.c_void_p)
+        assert g.lib.picgpu_upload_grid(g.h, 2048, 1024, 1024, ptr) == -6
+        assert g.lib.picgpu_upload_grid(g.h, 1291, 1291, 1291, ptr) == -6  # 2,151,685,171 cells
+        g.upload_grid(4, 4, 4, W.uniform_grid(4, 4, 4))
+        cells = np.zeros(3, np.int64)
+        st = np.zeros(1, np.int64)
+        assert g.lib.picgpu_upload_particles(g.h, (1 << 31) - 1, ptr, None, cells.ctypes.data_as(L.C.c_void_p), None,
+                                             st.ctypes.data_as(L.C.c_void_p), st.ctypes.data_as(L.C.c_void_p),
+                                             None) == -6
+        # the context stays usable
+        g.upload_particles(np.array([[0.3, 0.4, 0.55]]))
+        g.LocateAllParticlesInGrid()
+        assert g.n == 1 and (g.download_particles()["cell"] == [[1, 1, 2]]).all()
