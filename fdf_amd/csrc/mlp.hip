@@ -7,9 +7,10 @@
 // 2019, section 5.2, applied to GELU; the same function in training,
 // inference and the oracle) maps normalised physical
 // coordinates to the fractional logical position ((i+a1)/IM, (j+a2)/JM,
-// (k+a3)/KM).  Weights are BF16, products accumulate in FP32; layer 1 runs on
-// the VALU (K = 3), layers 2 and 3 and the large backward products on MFMA
-// (v_mfma_f32_32x32x8_bf16).  Training (forward, backward, Adam) runs on the
+// (k+a3)/KM).  Weights are BF16, products accumulate in FP32.  In training
+// layer 1 runs on the VALU (K = 3), layers 2 and 3 and the large backward
+// products on MFMA (v_mfma_f32_32x32x8_bf16); inference runs every layer on
+// MFMA (k_mlp_infer).  Training (forward, backward, Adam) runs on the
 // GPU on (position, located cell) pairs produced by the walking search.  The
 // guess only chooses where the walk starts (the slot of
 // InitializeTraversalParameters' fallback start, walkingsearch.c:603-619);
@@ -166,8 +167,8 @@ struct MlpNorm {
 };
 __device__ __forceinline__ float norm_axis(double x, double lo, double s) { return (float)((x - lo) * s - 1.0); }
 
-// Inference of 32 samples per wavefront, every layer on MFMA, the weights in
-// registers.  The layers run transposed (Z1^T = W1 X^T, Z2^T = W2 A1^T,
+// Inference of 32 samples per wavefront, every layer on MFMA; W1 and W2 in
+// registers, W3 and the layer-2 biases in LDS.  The layers run transposed (Z1^T = W1 X^T, Z2^T = W2 A1^T,
 // Y^T = W3 A2^T): lane l works on sample n = l%32 with h = l/32, and the
 // result register r of a 32-unit column tile t is hidden unit
 // 32t + 8(r/4) + 4h + r%4 of that sample, so a layer's results are already
