@@ -892,7 +892,8 @@ static int build_key_index(picgpu_ctx *c, const int32_t *guess) {
 // Moves the live particles into key order (the order of kidx[kcur]); the
 // index becomes the identity.  live_dev: the live count when it is still on
 // the device (live is then an upper bound, and c->n is set by the caller).
-static int compact_by_key(picgpu_ctx *c, int64_t live, int mode, const int32_t *live_dev = nullptr) {
+static int compact_by_key(picgpu_ctx *c, int64_t live, int mode, const int32_t *live_dev = nullptr,
+                          bool move_vw = true) {
   PG_CHECK(ensure_swarm(c, c->alt, c->n));
   int32_t *idx = c->kidx[c->kcur].as<int32_t>();
   const bool move32 = c->pid32_valid;  // the 32-bit PID copy moves with the swarm
@@ -900,7 +901,7 @@ static int compact_by_key(picgpu_ctx *c, int64_t live, int mode, const int32_t *
     int r = dispatch_real(c, [&](auto *tag) -> int {
       using R = std::remove_pointer_t<decltype(tag)>;
       PG_LAUNCH(c, "gather_swarm", nblocks(live, kBlock * kGatherPer), kBlock, k_gather_swarm<R>, live, live_dev, mode,
-                idx, c->sw.pos.as<double>(),
+                move_vw ? 1 : 0, idx, c->sw.pos.as<double>(),
                 c->sw.vel.as<R>(), c->sw.w.as<R>(), c->sw.cell.as<int32_t>(), c->sw.status.as<int32_t>(),
                 c->sw.pid.as<int64_t>(), c->sw.P.as<R>(), c->alt.pos.as<double>(), c->alt.vel.as<R>(),
                 c->alt.w.as<R>(), c->alt.cell.as<int32_t>(), c->alt.status.as<int32_t>(), c->alt.pid.as<int64_t>(),
@@ -1148,7 +1149,11 @@ static int do_step_fused(picgpu_ctx *c, double dt, unsigned flags, int step, boo
   if (c->steps_since_compact >= kCompactEvery) {
     // every survivor of a fused step is located; at the call's last step the
     // weights and statuses move too; the live count is the new index's total
-    PG_CHECK(compact_by_key(c, n, last ? GATHER_LAST : GATHER_STEP, nstart + nkeys + 1));
+    // the FP64 steps recompute every start state: between the steps of a
+    // call (and before the call's last pass writes them) weights and
+    // velocities are not read, unless the per-step field path carries them
+    const bool move_vw = wu || c->opt.precision == PICGPU_FP32;
+    PG_CHECK(compact_by_key(c, n, last ? GATHER_LAST : GATHER_STEP, nstart + nkeys + 1, move_vw));
     c->steps_since_compact = 0;
     c->compact_pending = false;
     *compacted = true;
