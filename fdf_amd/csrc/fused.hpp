@@ -1729,7 +1729,7 @@ __global__ __launch_bounds__(kBlock) void k_count_from_keys(int IM, int JM, int 
 // Physical reorder: dst[q] = src[idx[q]] for q < m.  Between the steps of one
 // call (GATHER_STEP) statuses are not moved: every particle is
 // ACTIVE_AND_LOCATED.  At the end of a call (GATHER_LAST) statuses are set
-// to ACTIVE_AND_LOCATED.  Weights and velocities move when move_vw is set:
+// to ACTIVE_AND_LOCATED.  Weights and velocities move when MOVE_VW is set:
 // between the steps of a call they hold the start states that an FP32 step
 // cannot recompute (uncertified faces) and the per-step field path reads;
 // the FP64 steps of a call recompute every start state and leave them behind.
@@ -1737,9 +1737,11 @@ __global__ __launch_bounds__(kBlock) void k_count_from_keys(int IM, int JM, int 
 // issued before the first store.
 enum GatherMode : int { GATHER_ALL = 0, GATHER_STEP = 1, GATHER_LAST = 2 };
 constexpr int kGatherPer = 4;
-template <typename R>
+// (a template parameter: a run-time flag there made the FP32 gather 40 %
+// slower, its loads no longer issued ahead of the stores).
+template <typename R, bool MOVE_VW>
 __global__ __launch_bounds__(kBlock) void k_gather_swarm(int64_t m_max, const int32_t *__restrict__ m_dev, int mode,
-                                                         int move_vw, const int32_t *__restrict__ idx,
+                                                         const int32_t *__restrict__ idx,
                                                          const double *__restrict__ pos, const R *__restrict__ vel,
                                                          const R *__restrict__ w, const int32_t *__restrict__ cell,
                                                          const int32_t *__restrict__ status,
@@ -1765,8 +1767,10 @@ __global__ __launch_bounds__(kBlock) void k_gather_swarm(int64_t m_max, const in
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       x[u][c] = pos[3 * p[u] + c];
-      v[u][c] = move_vw ? vel[3 * p[u] + c] : (R)0;
-      ww[u][c] = move_vw ? w[3 * p[u] + c] : (R)0;
+      if constexpr (MOVE_VW) {
+        v[u][c] = vel[3 * p[u] + c];
+        ww[u][c] = w[3 * p[u] + c];
+      }
     }
     ce[u] = cell[p[u]];
     st[u] = mode == GATHER_ALL ? status[p[u]] : ST_LOCATED;
@@ -1780,7 +1784,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_swarm(int64_t m_max, const in
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       pos2[3 * q + c] = x[u][c];
-      if (move_vw) {
+      if constexpr (MOVE_VW) {
         vel2[3 * q + c] = v[u][c];
         w2[3 * q + c] = ww[u][c];
       }

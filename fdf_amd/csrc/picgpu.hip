@@ -900,14 +900,17 @@ static int compact_by_key(picgpu_ctx *c, int64_t live, int mode, const int32_t *
   if (live > 0) {
     int r = dispatch_real(c, [&](auto *tag) -> int {
       using R = std::remove_pointer_t<decltype(tag)>;
-      PG_LAUNCH(c, "gather_swarm", nblocks(live, kBlock * kGatherPer), kBlock, k_gather_swarm<R>, live, live_dev, mode,
-                move_vw ? 1 : 0, idx, c->sw.pos.as<double>(),
-                c->sw.vel.as<R>(), c->sw.w.as<R>(), c->sw.cell.as<int32_t>(), c->sw.status.as<int32_t>(),
-                c->sw.pid.as<int64_t>(), c->sw.P.as<R>(), c->alt.pos.as<double>(), c->alt.vel.as<R>(),
-                c->alt.w.as<R>(), c->alt.cell.as<int32_t>(), c->alt.status.as<int32_t>(), c->alt.pid.as<int64_t>(),
-                c->alt.P.as<R>(), move32 ? c->sw.pid32.as<int32_t>() : (const int32_t *)nullptr,
-                c->alt.pid32.as<int32_t>());
-      return PICGPU_OK;
+      auto gather = [&](auto vw_tag) -> int {
+        PG_LAUNCH(c, "gather_swarm", nblocks(live, kBlock * kGatherPer), kBlock,
+                  (k_gather_swarm<R, decltype(vw_tag)::value>), live, live_dev, mode, idx, c->sw.pos.as<double>(),
+                  c->sw.vel.as<R>(), c->sw.w.as<R>(), c->sw.cell.as<int32_t>(), c->sw.status.as<int32_t>(),
+                  c->sw.pid.as<int64_t>(), c->sw.P.as<R>(), c->alt.pos.as<double>(), c->alt.vel.as<R>(),
+                  c->alt.w.as<R>(), c->alt.cell.as<int32_t>(), c->alt.status.as<int32_t>(), c->alt.pid.as<int64_t>(),
+                  c->alt.P.as<R>(), move32 ? c->sw.pid32.as<int32_t>() : (const int32_t *)nullptr,
+                  c->alt.pid32.as<int32_t>());
+        return PICGPU_OK;
+      };
+      return move_vw ? gather(std::true_type{}) : gather(std::false_type{});
     });
     PG_CHECK(r);
     PG_LAUNCH(c, "iota", nblocks(live), kBlock, k_iota, live, live_dev, idx);
