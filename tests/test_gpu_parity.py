@@ -372,9 +372,10 @@ def test_crowded_tiles_with_fast_particles(swarm_cls, oracle, precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_single_step_calls_match_one_call(swarm_cls, precision):
-    """Ten one-step calls (each continuing from the key index the previous
-    call left) give exactly what one ten-step call gives, with and without
-    deposition in between."""
+    """Twenty one-step calls (each continuing from the key index the previous
+    call left; the compaction every 16 steps falls inside the sequence) give
+    exactly what one twenty-step call gives, with and without deposition in
+    between."""
     _say("single_step_calls", precision)
     d = W.config(5, scale=0.02)
     d["pos"] = d["pos"][:30000]
@@ -387,13 +388,13 @@ def test_single_step_calls_match_one_call(swarm_cls, precision):
             g.LocateAllParticlesInGrid()
             g.InterpolateAllFieldsToSwarm()
             if split:
-                for s in range(10):
+                for s in range(20):
                     g.AdvanceSimulation(d["dt"] * 3, 1, rk2=True, deposit_cells=True,
-                                        deposit_faces=s % 3 == 2 or s == 9)
+                                        deposit_faces=s % 3 == 2 or s == 19)
             else:
-                g.AdvanceSimulation(d["dt"] * 3, 10, rk2=True, deposit_cells=True, deposit_faces=True)
+                g.AdvanceSimulation(d["dt"] * 3, 20, rk2=True, deposit_cells=True, deposit_faces=True)
             st = g.stats()
-            n_live = g.n  # after ten single-step calls the last compaction is still pending
+            n_live = g.n  # after the calls a compaction is still pending
             p = g.download_particles()
             assert n_live == st["n_particles"] == len(p["pid"]) == st["located"]
             outs.append((_by_pid(p), g.field(L.FIELD_KFACE), g.field(L.FIELD_P_AVERAGE), st["located"]))
