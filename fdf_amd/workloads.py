@@ -240,6 +240,17 @@ def ucat_taylor_green(coor, IM, JM, KM, xp=None):
     return to_node_slots(cells, IM, JM, KM, xp)
 
 
+def ucat_sine(coor, IM, JM, KM, xp=None):
+    """The finit = 1 field of SetLocalCartesianField_Vector
+    (AnalyticalSolution.c:36-40), (sin x, sin y, sin z), at the cell
+    centroids: the field LOG_INTERPOLATION_ERROR (logging.c:533-580) checks
+    the interpolated particle velocities against."""
+    xp = xp or _NumpyNS()
+    cc = cell_centroids(coor, IM, JM, KM)
+    cells = xp.stack([xp.sin(cc[..., 0]), xp.sin(cc[..., 1]), xp.sin(cc[..., 2])], axis=-1)
+    return to_node_slots(cells, IM, JM, KM, xp)
+
+
 def fourier_modes(seed, nmodes=64, kmax=8):
     rng = np.random.Generator(np.random.PCG64(seed))
     ks, amps, phis = [], [], []

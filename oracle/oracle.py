@@ -19,6 +19,22 @@ LOCATED, LOST = 1, 3
 MAX_TRAVERSAL = 1000
 
 
+def interpolation_error_percent(pos, vel, finit: int) -> float:
+    """LOG_INTERPOLATION_ERROR (logging.c:533-580): 100 |a - v|_2 / |a|_2 over
+    every component of every particle, with the analytical velocity a of
+    SetAnalyticalSolution (AnalyticalSolution.c:932-962): sin of each
+    position component for finit 1, (0, 0, 1) for finit 0."""
+    pos = np.asarray(pos, np.float64).reshape(-1, 3)
+    if finit == 1:
+        a = np.sin(pos)
+    elif finit == 0:
+        a = np.zeros_like(pos)
+        a[:, 2] = 1.0
+    else:
+        raise ValueError("finit 0 or 1")
+    return 100.0 * float(np.linalg.norm(a - np.asarray(vel, np.float64).reshape(-1, 3)) / np.linalg.norm(a))
+
+
 def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 

@@ -145,6 +145,53 @@ def test_interpolate(swarm_cls, oracle, grid, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_interpolation_error_diagnostic(swarm_cls, oracle, precision):
+    """The reference's known-answer diagnostic LOG_INTERPOLATION_ERROR
+    (logging.c:533-580 with AnalyticalSolution.c:932-962) on the HIP path:
+    the L2 error, in percent, of the interpolated particle velocities against
+    (sin x, sin y, sin z) when the cell field is that function at the
+    centroids (finit 1), and against (0, 0, 1) for the uniform field
+    (finit 0).  It equals the oracle's (FP64 bit for bit, FP32 within 1e-4
+    percent); the uniform field's error is zero; and on uniform grids it
+    falls by 2^1.5 per halving of h, the order of centre->corner + trilinear
+    with the boundary's one-sided corner averages."""
+    from oracle.oracle import interpolation_error_percent
+
+    _say("interpolation error diagnostic", precision)
+    L = 2.0
+    errs = {}
+    for finit, m, grid in ((1, 16, "uniform"), (1, 32, "uniform"), (1, 16, "perturbed"), (0, 16, "perturbed")):
+        coor = (W.perturbed_cube(m, m, m, seed=7) if grid == "perturbed" else W.uniform_grid(m, m, m)) * L
+        ucat = W.ucat_sine(coor, m, m, m) if finit == 1 else W.ucat_uniform(m, m, m)
+        pos = L * np.random.default_rng(m + finit).random((20000, 3))
+        cell, w, st, _ = oracle.locate(m, m, m, coor, pos)
+        vo = oracle.interp(m, m, m, oracle.center_to_corner(m, m, m, ucat), cell, w)
+        with swarm_cls(precision=precision) as g:
+            g.upload_grid(m, m, m, coor)
+            g.upload_ucat(ucat)
+            g.upload_particles(pos)
+            g.LocateAllParticlesInGrid()
+            g.InterpolateAllFieldsToSwarm()
+            out = _by_pid(g.download_particles())
+        # walks from cell (0,0,0) that cycle end LOST (the reference's start
+        # rule); the diagnostic is taken over the located particles
+        ok = st == 1
+        assert ok.mean() > 0.95
+        np.testing.assert_array_equal(out["status"], st)
+        np.testing.assert_array_equal(out["cell"][ok], cell[ok])
+        eg = interpolation_error_percent(out["pos"][ok], out["vel"][ok], finit)
+        eo = interpolation_error_percent(pos[ok], vo[ok], finit)
+        _say("finit", finit, grid, m, "error %", eg, "oracle", eo)
+        if precision == "fp64":
+            assert np.array_equal(out["vel"][ok], vo[ok]) and eg == eo
+        else:
+            assert abs(eg - eo) <= 1e-4
+        errs[(finit, m, grid)] = eg
+    assert errs[(0, 16, "perturbed")] <= (0.0 if precision == "fp64" else 1e-4)
+    assert 2.6 < errs[(1, 16, "uniform")] / errs[(1, 32, "uniform")] < 3.0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_advect_and_removal(swarm_cls, oracle, precision):
     _say("advect_remove", precision)
     IM, JM, KM, coor = GRIDS["warped"]()

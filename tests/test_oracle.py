@@ -240,6 +240,43 @@ def test_uniform_field_known_answer(oracle):
     corner = oracle.center_to_corner(IM, JM, KM, d["ucat"])
     vel = oracle.interp(IM, JM, KM, corner, cell, w)
     assert (vel == np.array([0.0, 0.0, 1.0])).all()
+    from oracle.oracle import interpolation_error_percent
+
+    assert interpolation_error_percent(d["pos"][st == 1], vel[st == 1], 0) == 0.0
+
+
+def _sine_error(oracle, m, interior, L=2.0):
+    """LOG_INTERPOLATION_ERROR's finit = 1 diagnostic on an m^3 cube of side L."""
+    from oracle.oracle import interpolation_error_percent
+
+    coor = W.uniform_grid(m, m, m) * L
+    ucat = W.ucat_sine(coor, m, m, m)
+    rng = np.random.default_rng(1)
+    pos = L * (0.25 + 0.5 * rng.random((20000, 3))) if interior else L * rng.random((20000, 3))
+    cell, w, st, _ = oracle.locate(m, m, m, coor, pos)
+    vel = oracle.interp(m, m, m, oracle.center_to_corner(m, m, m, ucat), cell, w)
+    ok = st == 1
+    assert ok.all()
+    return interpolation_error_percent(pos[ok], vel[ok], 1)
+
+
+def test_sine_field_interpolation_error_converges(oracle):
+    """LOG_INTERPOLATION_ERROR's finit = 1 case, (sin x, sin y, sin z) at the
+    cell centroids: centre->corner averaging plus trilinear interpolation is
+    second order inside the domain (the error falls 4x per halving of h), and
+    the one-sided corner averages of the boundary layer bring the whole
+    domain's L2 error to order 3/2 (2^1.5 = 2.83x).  A corner averaged over
+    the wrong cells, or weights taken from the wrong faces, would be first
+    order or worse."""
+    inner = [_sine_error(oracle, m, True) for m in (8, 16, 32)]
+    whole = [_sine_error(oracle, m, False) for m in (8, 16, 32)]
+    for e in (inner, whole):
+        assert all(a > b > 0 for a, b in zip(e, e[1:]))
+    for a, b in zip(inner, inner[1:]):
+        assert 3.8 < a / b < 4.2
+    for a, b in zip(whole, whole[1:]):
+        assert 2.6 < a / b < 3.0
+    assert inner[-1] < 0.1  # percent
 
 
 # ------------------------------------------------------------------ deposition
