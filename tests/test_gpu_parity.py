@@ -575,3 +575,44 @@ def test_steps_deposit_the_state_they_start_from(swarm_cls, oracle, case, precis
             a4, b4 = a.reshape(-1, 4), b.reshape(-1, 4)
             assert np.abs(a4[:, 0] - b4[:, 0]).max() <= FP32_TOL * max(1.0, np.abs(b4[:, 0]).max())
             assert np.abs(a4[:, 1:] - b4[:, 1:]).max() <= FP32_TOL * max(1.0, np.abs(b4[:, 1:]).max())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_equal_pids_are_each_deposited_once(swarm_cls, oracle, precision):
+    """A caller may upload equal PIDs (here a tenth of the PIDs distinct,
+    each shared by about ten particles).  The in-step deposition, which picks
+    each cell's particles by PID, sums every particle once: the faces equal
+    the oracle's deposition of the same state up to summation order (equal
+    PIDs leave the order among them open), and each direction's face counts
+    add up to the particles."""
+    _say("equal pids", precision)
+    IM, JM, KM = 16, 16, 8
+    coor = W.perturbed_cube(IM, JM, KM, seed=51, amp=0.15)
+    ucat = W.ucat_random_fourier(coor, IM, JM, KM, seed=52)
+    rng = np.random.default_rng(53)
+    npart = 20000
+    pos = 0.05 + 0.9 * rng.random((npart, 3))
+    pid = rng.integers(0, npart // 10, npart).astype(np.int64)
+    dt = 1.5 / (np.abs(ucat).max() * max(IM, JM, KM))
+
+    def run(n, faces_of_start):
+        with swarm_cls(precision=precision) as g:
+            g.upload_grid(IM, JM, KM, coor)
+            g.upload_ucat(ucat)
+            g.upload_particles(pos, pid=pid)
+            g.LocateAllParticlesInGrid()
+            g.InterpolateAllFieldsToSwarm()
+            g.AdvanceSimulation(dt, n, rk2=True, deposit_cells=False, deposit_faces=True,
+                                faces_of_start=faces_of_start)
+            return g.download_particles(), [g.field(f) for f in (L.FIELD_IFACE, L.FIELD_JFACE, L.FIELD_KFACE)]
+
+    before, _ = run(2, False)
+    _, faces = run(3, True)
+    nb = len(before["pid"])
+    ref = oracle.deposit_faces(IM, JM, KM, before["cell"], before["weight"], before["vel"], before["pid"])
+    tol = 1e-12 if precision == "fp64" else FP32_TOL
+    for a, b in zip(faces, ref):
+        a4, b4 = a.reshape(-1, 4), b.reshape(-1, 4)
+        assert abs(a4[:, 0].sum() - nb) <= tol * nb
+        assert np.abs(a4[:, 0] - b4[:, 0]).max() <= tol * max(1.0, np.abs(b4[:, 0]).max())
+        assert np.abs(a4[:, 1:] - b4[:, 1:]).max() <= tol * max(1.0, np.abs(b4[:, 1:]).max())
