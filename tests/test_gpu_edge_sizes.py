@@ -69,14 +69,12 @@ def _grid(IM, JM, KM, seed):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-@pytest.mark.parametrize("path", ["fused", "staged"])
-@pytest.mark.parametrize("faces_of_start", [False, True])
+@pytest.mark.parametrize("path,faces_of_start", [("fused", False), ("fused", True), ("staged", False)])
 def test_empty_swarm(swarm_cls, oracle, precision, path, faces_of_start):
     """No particles: locate, interpolation and three RK2 steps with cell and
     face deposition succeed and deposit nothing (zero counts, P averages and
-    faces, as the oracle gives for an empty swarm)."""
-    if path == "staged" and faces_of_start:
-        pytest.skip("faces_of_start is a fused-step option")
+    faces, as the oracle gives for an empty swarm).  faces_of_start (the
+    in-step deposition's faces) is an option of the fused step only."""
     _say("empty swarm", precision, path, faces_of_start)
     IM, JM, KM = 12, 10, 9
     coor, ucat, dt = _grid(IM, JM, KM, 41)
