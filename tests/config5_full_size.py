@@ -25,7 +25,7 @@ import os
 import sys
 
 import numpy as np
-import torch  # before the library: torch's HIP runtime is then the one both use, as in bench.py
+import torch  # before the library uses the GPU, as in bench.py (imported after, it finds no GPU)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:

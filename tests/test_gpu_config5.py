@@ -6,9 +6,10 @@ and the sample's parity bar.
 
 It runs as a child process.  The workload is generated on the device with
 torch (the 64-mode Fourier field over 67M cells takes minutes in numpy), and
-torch finds no GPU when it starts after the library has loaded the HIP
-runtime it was built against; the child imports torch first, as bench.py
-does.  The child's output is printed with the test's."""
+torch, first imported in a process where the library has already used the
+GPU (as the earlier GPU tests leave this one), reports that no HIP GPU is
+available; the child imports torch first, as bench.py does.  The child's
+output is printed with the test's."""
 import os
 import subprocess
 import sys
