@@ -120,6 +120,8 @@ _SIG = {
     "picgpu_locate_device_noalloc": (C.c_int, [_P, _I64, _P, _P, _P, _P, C.POINTER(PicgpuStats)]),
     "picgpu_grid_field_size": (C.c_int, [_P, C.c_int, C.POINTER(_I64)]),
     "picgpu_download_grid_field": (C.c_int, [_P, C.c_int, _P]),
+    "picgpu_particle_count_summary": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                                C.POINTER(_I64)]),
     "picgpu_get_stats": (C.c_int, [_P, C.POINTER(PicgpuStats)]),
     "picgpu_walk_stats_enable": (C.c_int, [_P, C.c_int]),
     "picgpu_timing_enable": (C.c_int, [_P, C.c_int]),

@@ -1726,6 +1726,39 @@ __global__ __launch_bounds__(kBlock) void k_count_from_keys(int IM, int JM, int 
   out[c] = kstart[key + 1] - kstart[key];
 }
 
+// The total and the largest of the per-cell counts, with the first node slot
+// holding the largest (CalculateParticleCountPerCell's VecSum and VecMax,
+// ParticleMotion.c:1021-1024; a cell (i,j,k) sits at node slot (i,j,k) and
+// the other slots hold 0).  out[0] += the counts; out[1] = the maximum over
+// cells of count << 33 | (2^33 - 1 - slot), so that the largest count wins
+// and, among equal counts, the smallest slot (fewer than 2^33 slots, counts
+// below 2^31).  Integer atomics: the result does not depend on their order.
+__global__ __launch_bounds__(kBlock) void k_count_summary(int IM, int JM, int KM, const int32_t *__restrict__ count,
+                                                          unsigned long long *__restrict__ out) {
+  constexpr unsigned long long kSlotMask = (1ull << 33) - 1;
+  const int64_t nc = (int64_t)IM * JM * KM;
+  unsigned long long sum = 0, best = 0;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t v = count[c];
+    int i, j, k;
+    unpack_cell((int32_t)c, IM, JM, i, j, k);
+    const unsigned long long slot = ((unsigned long long)k * (JM + 1) + j) * (IM + 1) + i;
+    const unsigned long long key = ((unsigned long long)(uint32_t)v << 33) | (kSlotMask - slot);
+    sum += (unsigned long long)(uint32_t)v;
+    best = key > best ? key : best;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sum += __shfl_xor(sum, off);
+    const unsigned long long o = __shfl_xor(best, off);
+    best = o > best ? o : best;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&out[0], sum);
+    atomicMax(&out[1], best);
+  }
+}
+
 // Physical reorder: dst[q] = src[idx[q]] for q < m.  Between the steps of one
 // call (GATHER_STEP) statuses are not moved: every particle is
 // ACTIVE_AND_LOCATED.  At the end of a call (GATHER_LAST) statuses are set

@@ -1401,6 +1401,41 @@ int picgpu_grid_field_size(picgpu_ctx *c, int which, int64_t *count) {
   return PICGPU_OK;
 }
 
+// The per-cell counts in cell_count: from the stage-wise deposition, or from
+// the bucket sizes of the key index after a fused step.
+static int ensure_cell_counts(picgpu_ctx *c) {
+  if (!c->sorted && !c->keyed) {
+    set_error("ParticleCount needs picgpu_scatter/picgpu_deposit_faces/picgpu_step first");
+    return PICGPU_E_STATE;
+  }
+  if (!c->sorted) {
+    PG_CHECK(c->cell_count.ensure(sizeof(int32_t) * (size_t)(c->ncell + 1)));
+    PG_LAUNCH(c, "count_from_keys", nblocks(c->ncell), kBlock, k_count_from_keys, c->IM, c->JM, c->KM,
+              tile_map(c), c->kstart[c->kcur].as<int32_t>(), c->cell_count.as<int32_t>());
+  }
+  return PICGPU_OK;
+}
+
+int picgpu_particle_count_summary(picgpu_ctx *c, double *total, double *max_count, int64_t *max_index) {
+  if (!c || !total || !max_count || !max_index) return PICGPU_E_ARG;
+  PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(need_grid(c));
+  PG_CHECK(ensure_cell_counts(c));
+  PG_CHECK(c->count_summary.ensure(2 * sizeof(unsigned long long)));
+  unsigned long long *d = c->count_summary.as<unsigned long long>();
+  PG_HIP(hipMemsetAsync(d, 0, 2 * sizeof(unsigned long long), c->stream));
+  PG_LAUNCH(c, "count_summary", std::min<int64_t>(nblocks(c->ncell), 2048), kBlock, k_count_summary, c->IM, c->JM,
+            c->KM, c->cell_count.as<int32_t>(), d);
+  unsigned long long h[2];
+  PG_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  PG_HIP(hipStreamSynchronize(c->stream));
+  constexpr unsigned long long kSlotMask = (1ull << 33) - 1;
+  *total = (double)h[0];
+  *max_count = (double)(h[1] >> 33);
+  *max_index = (int64_t)(kSlotMask - (h[1] & kSlotMask));
+  return PICGPU_OK;
+}
+
 int picgpu_download_grid_field(picgpu_ctx *c, int which, double *out) {
   if (!c || !out) return PICGPU_E_ARG;
   PG_HIP(hipSetDevice(c->device));
@@ -1415,17 +1450,8 @@ int picgpu_download_grid_field(picgpu_ctx *c, int which, double *out) {
     return PICGPU_OK;
   }
   if (which == PICGPU_FIELD_PARTICLE_COUNT) {
-    if (!c->sorted && !c->keyed) {
-      set_error("ParticleCount needs picgpu_scatter/picgpu_deposit_faces/picgpu_step first");
-      return PICGPU_E_STATE;
-    }
-    if (!c->sorted) {
-      // after a fused step: counts from the bucket sizes of the key index
-      PG_CHECK(c->cell_count.ensure(sizeof(int32_t) * (size_t)(c->ncell + 1)));
-      PG_LAUNCH(c, "count_from_keys", nblocks(c->ncell), kBlock, k_count_from_keys, c->IM, c->JM, c->KM,
-                tile_map(c), c->kstart[c->kcur].as<int32_t>(), c->cell_count.as<int32_t>());
-      PG_HIP(hipStreamSynchronize(c->stream));
-    }
+    PG_CHECK(ensure_cell_counts(c));
+    PG_HIP(hipStreamSynchronize(c->stream));
     std::vector<int32_t> h((size_t)c->ncell);
     PG_HIP(hipMemcpy(h.data(), c->cell_count.ptr, sizeof(int32_t) * (size_t)c->ncell, hipMemcpyDeviceToHost));
     memset(out, 0, sizeof(double) * (size_t)c->nnode);

@@ -140,6 +140,14 @@ class PicGpu:
         L.check(self.lib.picgpu_download_grid_field(self.h, which, _ptr(out)))
         return out
 
+    def particle_count_summary(self) -> tuple[float, float, int]:
+        """CalculateParticleCountPerCell's logged diagnostics (ParticleMotion.c:1021-1026):
+        (total counted, largest count in a cell, its first flat node-slot index),
+        reduced on the device."""
+        tot, mx, at = C.c_double(), C.c_double(), C.c_int64()
+        L.check(self.lib.picgpu_particle_count_summary(self.h, C.byref(tot), C.byref(mx), C.byref(at)))
+        return tot.value, mx.value, at.value
+
     def stats(self) -> dict:
         st = L.PicgpuStats()
         L.check(self.lib.picgpu_get_stats(self.h, C.byref(st)))

@@ -206,6 +206,13 @@ int picgpu_step(picgpu_ctx *ctx, double dt, int nsteps, unsigned flags);
 /* ---- grid outputs ---- */
 int picgpu_grid_field_size(picgpu_ctx *ctx, int which, int64_t *count);
 int picgpu_download_grid_field(picgpu_ctx *ctx, int which, double *out);
+/* The diagnostics CalculateParticleCountPerCell logs (ParticleMotion.c:1021-1026:
+ * VecSum and VecMax of ParticleCount), reduced on the device instead of read
+ * from a downloaded field: *total = the particles counted, *max_count = the
+ * largest count in a cell, *max_index = the first node slot holding it, in
+ * the flat node-slot order of PICGPU_FIELD_PARTICLE_COUNT (the index VecMax
+ * reports).  Needs the counts as that field does (PICGPU_E_STATE before). */
+int picgpu_particle_count_summary(picgpu_ctx *ctx, double *total, double *max_count, int64_t *max_index);
 
 /* ---- statistics and timing ---- */
 int picgpu_get_stats(picgpu_ctx *ctx, picgpu_stats *st);

@@ -616,3 +616,48 @@ def test_equal_pids_are_each_deposited_once(swarm_cls, oracle, precision):
         assert abs(a4[:, 0].sum() - nb) <= tol * nb
         assert np.abs(a4[:, 0] - b4[:, 0]).max() <= tol * max(1.0, np.abs(b4[:, 0]).max())
         assert np.abs(a4[:, 1:] - b4[:, 1:]).max() <= tol * max(1.0, np.abs(b4[:, 1:]).max())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", ["fused", "staged", "scatter", "empty"])
+def test_particle_count_summary(swarm_cls, precision, case):
+    """CalculateParticleCountPerCell's logged diagnostics (ParticleMotion.c:1021-1026:
+    VecSum and VecMax of ParticleCount) reduced on the device equal numpy's on
+    the downloaded field: the total, the largest count and the first flat
+    node slot holding it.  Two cells hold the largest count (40 particles
+    each), so the first of them must be reported; 500 other cells hold one
+    particle each.  Counts after a fused step, a staged step, the stage-wise
+    scatter, and an empty swarm (0, 0, slot 0)."""
+    _say("count summary", case, precision)
+    IM, JM, KM = 16, 12, 10
+    coor = W.uniform_grid(IM, JM, KM)
+    ucat = W.ucat_uniform(IM, JM, KM, (0.0, 0.0, 1.0))
+    rng = np.random.default_rng(61)
+    h = np.array([1.0 / IM, 1.0 / JM, 1.0 / KM])
+    cells = [(3, 4, 2)] * 40 + [(11, 9, 7)] * 40
+    flat = rng.choice(IM * JM * KM, 520, replace=False)
+    others = [(int(f % IM), int(f // IM % JM), int(f // (IM * JM))) for f in flat]
+    cells += [c for c in others if c not in ((3, 4, 2), (11, 9, 7))][:500]
+    cells = np.array(cells, np.float64)
+    pos = (cells + 0.3 + 0.4 * rng.random(cells.shape)) * h
+    if case == "empty":
+        pos = np.zeros((0, 3))
+    with swarm_cls(precision=precision) as g:
+        g.upload_grid(IM, JM, KM, coor)
+        g.upload_ucat(ucat)
+        g.upload_particles(pos)
+        g.LocateAllParticlesInGrid()
+        g.InterpolateAllFieldsToSwarm()
+        if case == "scatter":
+            g.ScatterAllParticleFieldsToEulerFields()
+        else:
+            g.AdvanceSimulation(1e-9, 1, rk2=True, deposit_cells=True, staged=case == "staged")
+        total, mx, at = g.particle_count_summary()
+        cnt = g.field(L.FIELD_PARTICLE_COUNT)
+    assert cnt.shape[0] == (IM + 1) * (JM + 1) * (KM + 1)
+    assert (total, mx, at) == (cnt.sum(), cnt.max(), int(np.argmax(cnt)))
+    if case == "empty":
+        assert (total, mx, at) == (0.0, 0.0, 0)
+    else:
+        assert (total, mx) == (580.0, 40.0)
+        assert at == (2 * (JM + 1) + 4) * (IM + 1) + 3  # cell (3,4,2), before (11,9,7)
