@@ -661,3 +661,62 @@ def test_particle_count_summary(swarm_cls, precision, case):
     else:
         assert (total, mx) == (580.0, 40.0)
         assert at == (2 * (JM + 1) + 4) * (IM + 1) + 3  # cell (3,4,2), before (11,9,7)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_reference_unit_test_point_near_the_upper_face(swarm_cls, oracle, precision):
+    """The expectation of the reference's one unit test (old/test_walkingsearch.c:103-128):
+    in a one-cell unit grid the point (0.9999999999999, 0.5, 0.5), 1e-13 from
+    the upper i-face, is found in cell (0,0,0); weights as the oracle's (FP64
+    bit for bit)."""
+    _say("reference unit test", precision)
+    coor = W.uniform_grid(1, 1, 1)
+    pos = np.array([[0.9999999999999, 0.5, 0.5]])
+    c, w, st, _ = oracle.locate(1, 1, 1, coor, pos)
+    assert st[0] == L.ACTIVE_AND_LOCATED and (c[0] == 0).all()
+    with swarm_cls(precision=precision) as g:
+        g.upload_grid(1, 1, 1, coor)
+        g.upload_particles(pos)
+        g.LocateAllParticlesInGrid()
+        d = g.download_particles()
+    assert d["status"][0] == L.ACTIVE_AND_LOCATED
+    np.testing.assert_array_equal(d["cell"], c)
+    if precision == "fp64":
+        assert np.array_equal(d["weight"], w)
+    else:
+        assert np.abs(d["weight"] - w).max() < FP32_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_runs_are_bit_identical(swarm_cls, precision):
+    """Two runs of one crowded workload (about 29 particles per cell, 7,500
+    per 8x8x4 tile, more than the step kernel stages in LDS; slots are taken
+    by atomics in whatever order the hardware runs them, and the swarm is
+    compacted once in the 20 steps) give the same
+    particles, per PID, and the same counts, P averages and faces, bit for
+    bit: deposits are sums in PID order, with no float atomics."""
+    _say("bit-identical runs", precision)
+    IM, JM, KM = 16, 16, 8
+    coor = W.perturbed_cube(IM, JM, KM, seed=71, amp=0.15)
+    ucat = W.ucat_random_fourier(coor, IM, JM, KM, seed=72)
+    pos = 0.05 + 0.9 * np.random.default_rng(73).random((60000, 3))
+    dt = 1.0 / (np.abs(ucat).max() * max(IM, JM, KM))
+    fields = (L.FIELD_PARTICLE_COUNT, L.FIELD_P_AVERAGE, L.FIELD_IFACE, L.FIELD_JFACE, L.FIELD_KFACE)
+
+    def run():
+        with swarm_cls(precision=precision) as g:
+            g.upload_grid(IM, JM, KM, coor)
+            g.upload_ucat(ucat)
+            g.upload_particles(pos)
+            g.LocateAllParticlesInGrid()
+            g.InterpolateAllFieldsToSwarm()
+            g.AdvanceSimulation(dt, 20, rk2=True, deposit_cells=True, deposit_faces=True)
+            return _by_pid(g.download_particles()), [g.field(f) for f in fields]
+
+    a, fa = run()
+    b, fb = run()
+    assert 0 < len(a["pid"]) < pos.shape[0]
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    for x, y in zip(fa, fb):
+        assert np.array_equal(x, y)
