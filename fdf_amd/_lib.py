@@ -122,6 +122,7 @@ _SIG = {
     "picgpu_download_grid_field": (C.c_int, [_P, C.c_int, _P]),
     "picgpu_particle_count_summary": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                                 C.POINTER(_I64)]),
+    "picgpu_interpolation_error": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
     "picgpu_get_stats": (C.c_int, [_P, C.POINTER(PicgpuStats)]),
     "picgpu_walk_stats_enable": (C.c_int, [_P, C.c_int]),
     "picgpu_timing_enable": (C.c_int, [_P, C.c_int]),

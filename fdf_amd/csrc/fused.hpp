@@ -1759,6 +1759,40 @@ __global__ __launch_bounds__(kBlock) void k_count_summary(int IM, int JM, int KM
   }
 }
 
+// The two sums of LOG_INTERPOLATION_ERROR (logging.c:533-580) over the
+// components q < 3n of the swarm's positions and velocities: the analytical
+// velocity a of SetAnalyticalSolution (AnalyticalSolution.c:932-962) is the
+// sine of each position component (finit 1) or (0, 0, 1) (finit 0);
+// part[2b] = block b's sum of (a - v)^2, part[2b + 1] its sum of a^2.  Each
+// thread strides over a fixed set of components and the block sums in a
+// fixed tree, so that a grid of a given size gives the same sums every run.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_interp_error_sums(int64_t n, int finit, const double *__restrict__ pos,
+                                                              const R *__restrict__ vel, double *__restrict__ part) {
+  __shared__ double se[kBlock], sa[kBlock];
+  double e = 0.0, aa = 0.0;
+  for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < 3 * n; q += (int64_t)gridDim.x * kBlock) {
+    const double a = finit == 1 ? sin(pos[q]) : (q % 3 == 2 ? 1.0 : 0.0);
+    const double d = a - (double)vel[q];
+    e += d * d;
+    aa += a * a;
+  }
+  se[threadIdx.x] = e;
+  sa[threadIdx.x] = aa;
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      se[threadIdx.x] += se[threadIdx.x + h];
+      sa[threadIdx.x] += sa[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = se[0];
+    part[2 * blockIdx.x + 1] = sa[0];
+  }
+}
+
 // Physical reorder: dst[q] = src[idx[q]] for q < m.  Between the steps of one
 // call (GATHER_STEP) statuses are not moved: every particle is
 // ACTIVE_AND_LOCATED.  At the end of a call (GATHER_LAST) statuses are set

@@ -1436,6 +1436,33 @@ int picgpu_particle_count_summary(picgpu_ctx *c, double *total, double *max_coun
   return PICGPU_OK;
 }
 
+int picgpu_interpolation_error(picgpu_ctx *c, int finit, double *percent) {
+  if (!c || !percent || (finit != 0 && finit != 1)) return PICGPU_E_ARG;
+  PG_HIP(hipSetDevice(c->device));
+  PG_CHECK(settle_swarm(c));
+  const int64_t n = c->n;
+  const int64_t nb = std::max<int64_t>(1, std::min<int64_t>(nblocks(3 * n), 1024));
+  PG_CHECK(c->count_summary.ensure(sizeof(double) * 2 * (size_t)nb));
+  double *part = c->count_summary.as<double>();
+  int r = dispatch_real(c, [&](auto *tag) -> int {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    PG_LAUNCH(c, "interp_error_sums", nb, kBlock, k_interp_error_sums<R>, n, finit, c->sw.pos.as<double>(),
+              c->sw.vel.as<R>(), part);
+    return PICGPU_OK;
+  });
+  PG_CHECK(r);
+  std::vector<double> h(2 * (size_t)nb);
+  PG_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+  PG_HIP(hipStreamSynchronize(c->stream));
+  double e = 0.0, a = 0.0;
+  for (int64_t b = 0; b < nb; ++b) {
+    e += h[2 * b];
+    a += h[2 * b + 1];
+  }
+  *percent = 100.0 * (sqrt(e) / sqrt(a));  // 0/0 (no particles) gives NaN, as the reference's does
+  return PICGPU_OK;
+}
+
 int picgpu_download_grid_field(picgpu_ctx *c, int which, double *out) {
   if (!c || !out) return PICGPU_E_ARG;
   PG_HIP(hipSetDevice(c->device));

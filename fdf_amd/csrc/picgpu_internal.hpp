@@ -177,7 +177,7 @@ struct picgpu_ctx {
 
   // per-cell / per-face outputs
   picgpu::DevBuf cell_count;  // int32 [ncell+1]
-  picgpu::DevBuf count_summary;  // 2 x u64: k_count_summary's sum and packed maximum
+  picgpu::DevBuf count_summary;  // k_count_summary's two u64, or k_interp_error_sums's block sums
   picgpu::DevBuf cell_start;  // int32 [ncell+1]
   picgpu::DevBuf pavg;        // real [ncell]
   picgpu::DevBuf face[3];     // real [nface_d][4]

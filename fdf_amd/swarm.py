@@ -148,6 +148,14 @@ class PicGpu:
         L.check(self.lib.picgpu_particle_count_summary(self.h, C.byref(tot), C.byref(mx), C.byref(at)))
         return tot.value, mx.value, at.value
 
+    def LOG_INTERPOLATION_ERROR(self, finit: int) -> float:
+        """The reference's interpolation-error diagnostic (logging.c:533-580), in
+        percent, reduced on the device: finit 1 compares with sin of the
+        positions, finit 0 with (0, 0, 1)."""
+        pct = C.c_double()
+        L.check(self.lib.picgpu_interpolation_error(self.h, finit, C.byref(pct)))
+        return pct.value
+
     def stats(self) -> dict:
         st = L.PicgpuStats()
         L.check(self.lib.picgpu_get_stats(self.h, C.byref(st)))

@@ -213,6 +213,15 @@ int picgpu_download_grid_field(picgpu_ctx *ctx, int which, double *out);
  * the flat node-slot order of PICGPU_FIELD_PARTICLE_COUNT (the index VecMax
  * reports).  Needs the counts as that field does (PICGPU_E_STATE before). */
 int picgpu_particle_count_summary(picgpu_ctx *ctx, double *total, double *max_count, int64_t *max_index);
+/* LOG_INTERPOLATION_ERROR (logging.c:533-580), the reference's known-answer
+ * check of the interpolation: *percent = 100 |a - v|_2 / |a|_2 over every
+ * component of every particle in the swarm (any status), v the particles'
+ * velocity as last interpolated, a the analytical velocity of
+ * SetAnalyticalSolution (AnalyticalSolution.c:932-962): sin of each position
+ * component for finit 1, (0, 0, 1) for finit 0 (the two the reference
+ * defines; others: PICGPU_E_ARG).  Reduced on the device in a fixed order;
+ * no particles give NaN (0/0), as the reference's norms do. */
+int picgpu_interpolation_error(picgpu_ctx *ctx, int finit, double *percent);
 
 /* ---- statistics and timing ---- */
 int picgpu_get_stats(picgpu_ctx *ctx, picgpu_stats *st);

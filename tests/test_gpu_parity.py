@@ -158,12 +158,12 @@ def test_interpolation_error_diagnostic(swarm_cls, oracle, precision):
     from oracle.oracle import interpolation_error_percent
 
     _say("interpolation error diagnostic", precision)
-    L = 2.0
+    side = 2.0
     errs = {}
     for finit, m, grid in ((1, 16, "uniform"), (1, 32, "uniform"), (1, 16, "perturbed"), (0, 16, "perturbed")):
-        coor = (W.perturbed_cube(m, m, m, seed=7) if grid == "perturbed" else W.uniform_grid(m, m, m)) * L
+        coor = (W.perturbed_cube(m, m, m, seed=7) if grid == "perturbed" else W.uniform_grid(m, m, m)) * side
         ucat = W.ucat_sine(coor, m, m, m) if finit == 1 else W.ucat_uniform(m, m, m)
-        pos = L * np.random.default_rng(m + finit).random((20000, 3))
+        pos = side * np.random.default_rng(m + finit).random((20000, 3))
         cell, w, st, _ = oracle.locate(m, m, m, coor, pos)
         vo = oracle.interp(m, m, m, oracle.center_to_corner(m, m, m, ucat), cell, w)
         with swarm_cls(precision=precision) as g:
@@ -172,9 +172,14 @@ def test_interpolation_error_diagnostic(swarm_cls, oracle, precision):
             g.upload_particles(pos)
             g.LocateAllParticlesInGrid()
             g.InterpolateAllFieldsToSwarm()
+            ed = g.LOG_INTERPOLATION_ERROR(finit)  # the library's own, over the whole swarm
             out = _by_pid(g.download_particles())
+        # the device reduction equals the formula over every particle (LOST
+        # ones included, with the zero velocity interpolation gives them)
+        ea = interpolation_error_percent(out["pos"], out["vel"], finit)
+        assert abs(ed - ea) <= 1e-12 + 1e-10 * ea
         # walks from cell (0,0,0) that cycle end LOST (the reference's start
-        # rule); the diagnostic is taken over the located particles
+        # rule); the comparisons below are taken over the located particles
         ok = st == 1
         assert ok.mean() > 0.95
         np.testing.assert_array_equal(out["status"], st)
@@ -189,6 +194,13 @@ def test_interpolation_error_diagnostic(swarm_cls, oracle, precision):
         errs[(finit, m, grid)] = eg
     assert errs[(0, 16, "perturbed")] <= (0.0 if precision == "fp64" else 1e-4)
     assert 2.6 < errs[(1, 16, "uniform")] / errs[(1, 32, "uniform")] < 3.0
+    with swarm_cls(precision=precision) as g:
+        g.upload_grid(4, 4, 4, W.uniform_grid(4, 4, 4))
+        g.upload_particles(np.zeros((0, 3)))
+        assert np.isnan(g.LOG_INTERPOLATION_ERROR(1))  # 0/0, as the reference's norms give
+        with pytest.raises(L.PicgpuError) as e:
+            g.LOG_INTERPOLATION_ERROR(2)  # the reference defines finit 0 and 1 only
+        assert e.value.code == -1
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
